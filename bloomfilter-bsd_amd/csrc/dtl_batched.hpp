@@ -1,0 +1,143 @@
+// dtl_batched.hpp -- one persistent launch that probes many filters, each
+// with its own probe batch (the semi-join pattern: thousands of per-partition
+// filters of uneven size).
+//
+// Work items are tiles of kTileKeys keys; every task's keys form a contiguous
+// run of global tiles.  Workgroups grab runs of `chunk` consecutive tiles from
+// an atomic counter until the counter passes the last tile, so large tasks are
+// spread over the whole GPU and small ones do not leave CUs idle.  Filters up
+// to `lds_max` bytes are staged in LDS once per run of tiles of the same task;
+// larger filters are probed from global memory (L2 / Infinity Cache / HBM).
+// Output is the same bitmap + per-tile count as the single-filter kernels, so
+// the compaction phases are shared.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dtl_internal.hpp"
+#include "dtl_kernels.hpp"
+
+namespace dtlh {
+
+template <class C, int BT>
+__global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __restrict__ tasks,
+                                                          const uint32_t* __restrict__ tile_task,
+                                                          uint32_t total_tiles, uint32_t chunk,
+                                                          uint32_t* __restrict__ work_counter,
+                                                          uint32_t lds_max,
+                                                          uint64_t* __restrict__ bits,
+                                                          uint32_t* __restrict__ tile_cnt) {
+  using W = typename C::word_t;
+  constexpr int ROUNDS = kTileKeys / BT;
+  constexpr int U = ROUNDS < kGlobalUnroll ? ROUNDS : kGlobalUnroll;
+  extern __shared__ uint4 lds_filter[];
+  __shared__ uint32_t red[BT / 64];
+  __shared__ uint32_t s_start;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const W* lds = reinterpret_cast<const W*>(lds_filter);
+  uint32_t staged = 0xFFFFFFFFu;
+  for (;;) {
+    if (tid == 0) s_start = atomicAdd(work_counter, chunk);
+    __syncthreads();
+    const uint32_t g0 = s_start;
+    __syncthreads();
+    if (g0 >= total_tiles) break;
+    const uint32_t g1 = min(g0 + chunk, total_tiles);
+    for (uint32_t g = g0; g < g1; ++g) {
+      const uint32_t ti = tile_task[g];
+      const BatchTaskDev& T = tasks[ti];
+      const uint32_t* keys = T.keys;
+      const uint32_t n = T.n;
+      AddrArgs addr;
+      addr.magic = T.addr_magic;
+      addr.mshift = T.addr_mshift;
+      addr.divisor = T.addr_divisor;
+      addr.post = T.addr_post;
+      const uint32_t fbytes = T.filter_bytes;
+      const W* filter = static_cast<const W*>(T.words);
+      const bool use_lds = fbytes <= lds_max;
+      if (use_lds && ti != staged) {
+        const uint32_t n16 = fbytes / 16;
+        const uint4* src = reinterpret_cast<const uint4*>(filter);
+        for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
+        uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds_filter);
+        const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
+        for (uint32_t i = n16 * 4 + tid; i < fbytes / 4; i += BT) dst32[i] = src32[i];
+        staged = ti;
+        __syncthreads();
+      }
+      const uint64_t local_base = static_cast<uint64_t>(g - T.tile_begin) * kTileKeys;
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int r0 = 0; r0 < ROUNDS; r0 += U) {
+        uint32_t key[U];
+        bool valid[U];
+        bool hit[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint64_t i = local_base + static_cast<uint64_t>(r0 + u) * BT + tid;
+          valid[u] = i < n;
+          key[u] = load_key(keys, valid[u] ? i : n - 1);  // tasks have n >= 1
+        }
+        if (use_lds) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t b = block_of(key[u] * kMul.v[0], addr);
+            hit[u] = test_ptr<C>(key[u], lds + static_cast<uint32_t>(b * C::w)) & valid[u];
+          }
+        } else {
+          W blk[U][C::w];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t b = block_of(key[u] * kMul.v[0], addr);
+            load_block<C>(filter + static_cast<uint64_t>(b) * C::w, blk[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) hit[u] = test_regs<C>(key[u], blk[u]) & valid[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint64_t m = __ballot(hit[u]);
+          if (lane == 0)
+            store_bits(bits, static_cast<uint64_t>(g) * kTileWords + ((r0 + u) * BT) / 64 + wave, m);
+          cnt += __popcll(m);
+        }
+      }
+      const uint32_t total = block_sum_waves<BT>(cnt, red);
+      if (tid == 0) tile_cnt[g] = total;
+      __syncthreads();
+    }
+  }
+}
+
+template <class C, int BT>
+hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_counter, uint32_t chunk) {
+  auto fn = k_bbf_probe_batched<C, BT>;
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)attr;
+  const uint32_t lds_bytes = (L.lds_max_bytes + 15u) & ~15u;
+  uint32_t grid = L.grid;
+  if (grid == 0) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BT, lds_bytes);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    grid = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(L.cu_count > 0 ? L.cu_count : 1);
+  }
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(BT), lds_bytes, L.stream, L.tasks, L.tile_task,
+                     L.total_tiles, chunk, work_counter, L.lds_max_bytes, L.bits, L.tile_cnt);
+  return hipGetLastError();
+}
+
+// `work_counter` must be zero when the launch starts; the caller clears it
+// with a kernel on the same stream (graph-capturable).
+template <class C>
+hipError_t launch_probe_batched(const BbfBatchLaunch& L) {
+  const uint32_t chunk = L.chunk ? L.chunk : 1u;
+  switch (L.threads) {
+    case 1024: return launch_probe_batched_bt<C, 1024>(L, L.work_counter, chunk);
+    default: return launch_probe_batched_bt<C, 256>(L, L.work_counter, chunk);
+  }
+}
+
+}  // namespace dtlh

@@ -1,0 +1,434 @@
+// dtl_capi.cpp -- the C ABI of libdtl_hip (declared in include/dtl_hip.h).
+//
+// Host-side sizing and validation mirror the reference's constructors; all
+// filter arithmetic runs in the gfx950 kernels.  There is no CPU fallback:
+// without a usable GPU every entry point that would launch work returns
+// DTL_ENODEVICE / DTL_EHIP.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <random>
+#include <vector>
+
+#include "dtl_hip.h"
+#include "dtl_internal.hpp"
+
+namespace dtlh {
+
+// ---------------------------------------------------------------------------
+// registry
+// ---------------------------------------------------------------------------
+const std::vector<BbfEntry>& bbf_registry() {
+  static const std::vector<BbfEntry> reg = [] {
+    std::vector<BbfEntry> v;
+    dtl_reg_u32_w1(v);
+    dtl_reg_u32_w2(v);
+    dtl_reg_u32_w4(v);
+    dtl_reg_u32_w8(v);
+    dtl_reg_u32_w16(v);
+    dtl_reg_u64_w1(v);
+    dtl_reg_u64_w2(v);
+    dtl_reg_u64_w4(v);
+    dtl_reg_u64_w8(v);
+    dtl_reg_u64_w16(v);
+    dtl_reg_z32_w4(v);
+    dtl_reg_z32_w8(v);
+    dtl_reg_z32_w16(v);
+    dtl_reg_z64_w4(v);
+    dtl_reg_z64_w8(v);
+    dtl_reg_z64_w16(v);
+    return v;
+  }();
+  return reg;
+}
+
+const BbfEntry* bbf_find(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k) {
+  for (const BbfEntry& e : bbf_registry())
+    if (e.word_bits == word_bits && e.w == w && e.s == s && e.z == z && e.k == k) return &e;
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// sizing helpers (math.hpp:13-46, div.hpp:29-57, block_addressing_logic.hpp)
+// ---------------------------------------------------------------------------
+static bool is_pow2(uint64_t x) { return x == 1 ? true : (x && ((x & (x - 1)) == 0)); }
+static uint64_t next_pow2(uint64_t v) { return 1ull << (64 - __builtin_clzll(v - 1)); }
+static uint32_t floor_log2(uint64_t v) { return 63u - static_cast<uint32_t>(__builtin_clzll(v)); }
+
+struct Magic {
+  uint32_t magic, shift, divisor;
+};
+
+// Smallest d >= max(n, 2), d not a power of two, whose round-up magic number
+// needs no add step (libdivide's "algorithm 1": e = d - 2^(32+L) mod d < 2^L).
+static bool cheap_magic(uint32_t n, Magic* out) {
+  const uint32_t d_max = 0xFFFFFFFFu - 2u;
+  for (uint32_t d = std::max(n, 2u); d <= d_max; ++d) {
+    if ((d & (d - 1)) == 0) continue;
+    const uint32_t L = floor_log2(d);
+    const unsigned __int128 p = static_cast<unsigned __int128>(1) << (32 + L);
+    const uint64_t q = static_cast<uint64_t>(p / d);
+    const uint64_t r = static_cast<uint64_t>(p % d);
+    if (d - r < (1ull << L)) {
+      out->magic = static_cast<uint32_t>(q + 1);
+      out->shift = L;
+      out->divisor = d;
+      return true;
+    }
+  }
+  return false;
+}
+
+struct Addressing {
+  uint32_t mode, block_cnt, shift, magic, mshift, mask;
+};
+
+static int make_addressing(uint64_t m, uint64_t unit_bits, Addressing* a) {
+  const uint64_t max_blocks = 1ull << 31;
+  const uint64_t desired = (m + unit_bits - 1) / unit_bits;
+  std::memset(a, 0, sizeof(*a));
+  a->mode = is_pow2(m) ? DTL_ADDR_POW2 : DTL_ADDR_MAGIC;
+  if (desired > max_blocks) return DTL_ELOGIC;
+  if (desired < 2) return DTL_EINVAL;  // a single block is undefined in the reference
+  if (a->mode == DTL_ADDR_POW2) {
+    const uint64_t bc = next_pow2(desired);
+    if (next_pow2(bc) > max_blocks) return DTL_ELOGIC;
+    a->block_cnt = static_cast<uint32_t>(bc);
+    a->shift = 32u - floor_log2(bc);
+    a->mask = static_cast<uint32_t>(bc - 1);
+  } else {
+    Magic mg;
+    if (!cheap_magic(static_cast<uint32_t>(desired), &mg)) return DTL_ELOGIC;
+    if (next_pow2(mg.divisor) > max_blocks) return DTL_ELOGIC;
+    a->block_cnt = mg.divisor;
+    a->magic = mg.magic;
+    a->mshift = mg.shift;
+    a->mask = static_cast<uint32_t>(next_pow2(mg.divisor) - 1);
+  }
+  return DTL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device info + per-stream scratch
+// ---------------------------------------------------------------------------
+struct DeviceInfo {
+  int cu_count = 0;
+  uint32_t max_lds = 0;
+};
+
+static int current_device_info(DeviceInfo* out, int* dev_out) {
+  static std::mutex mu;
+  static std::map<int, DeviceInfo> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DTL_ENODEVICE;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(dev);
+  if (it == cache.end()) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return DTL_ENODEVICE;
+    DeviceInfo di;
+    di.cu_count = prop.multiProcessorCount;
+    size_t lds = prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor > lds) lds = prop.maxSharedMemoryPerMultiProcessor;
+    di.max_lds = static_cast<uint32_t>(std::min<size_t>(lds, 160 * 1024));
+    it = cache.emplace(dev, di).first;
+  }
+  *out = it->second;
+  if (dev_out) *dev_out = dev;
+  return DTL_OK;
+}
+
+int current_device(int* dev, int* cu_count, uint32_t* max_lds) {
+  DeviceInfo di;
+  const int rc = current_device_info(&di, dev);
+  if (rc != DTL_OK) return rc;
+  if (cu_count) *cu_count = di.cu_count;
+  if (max_lds) *max_lds = di.max_lds;
+  return DTL_OK;
+}
+
+struct Scratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Scratch device memory per (device, stream), grown on demand.  Growth
+// synchronizes the stream before freeing the old buffer; a stream that is
+// being captured into a graph must have been warmed up with a call of at
+// least the same size.
+static int get_scratch(int dev, hipStream_t stream, size_t bytes, void** out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, Scratch> cache;
+  std::lock_guard<std::mutex> g(mu);
+  Scratch& s = cache[{dev, stream}];
+  if (s.bytes < bytes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+      return DTL_ERUNTIME;
+    if (s.ptr) {
+      if (hipStreamSynchronize(stream) != hipSuccess) return DTL_EHIP;
+      (void)hipFree(s.ptr);
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
+    if (hipMalloc(&s.ptr, want) != hipSuccess) {
+      s.ptr = nullptr;
+      return DTL_ENOMEM;
+    }
+    s.bytes = want;
+  }
+  *out = s.ptr;
+  return DTL_OK;
+}
+
+static int hip_status(hipError_t e) { return e == hipSuccess ? DTL_OK : DTL_EHIP; }
+
+static void fill_addr(const dtl_bbf_desc* d, uint32_t* magic, uint32_t* mshift, uint32_t* divisor,
+                      uint32_t* post) {
+  if (d->addr_mode == DTL_ADDR_POW2) {
+    *magic = 0;
+    *mshift = 0;
+    *divisor = 0;
+    *post = d->block_shift;
+  } else {
+    *magic = d->magic;
+    *mshift = d->magic_shift;
+    *divisor = d->block_cnt;
+    *post = 0;
+  }
+}
+
+static bool desc_ok(const dtl_bbf_desc* d) {
+  return d && d->block_cnt >= 2 && d->word_cnt == static_cast<uint64_t>(d->block_cnt) * d->w &&
+         (d->word_bits == 32 || d->word_bits == 64);
+}
+
+static uint32_t block_bytes(const dtl_bbf_desc* d) { return d->w * d->word_bits / 8; }
+
+static bool aligned_to(const void* p, uint32_t a) {
+  return (reinterpret_cast<uintptr_t>(p) % a) == 0;
+}
+
+}  // namespace dtlh
+
+using namespace dtlh;
+
+extern "C" {
+
+int dtl_version(void) { return 100; }
+
+const char* dtl_status_string(int s) {
+  switch (s) {
+    case DTL_OK: return "ok";
+    case DTL_EINVAL: return "invalid argument";
+    case DTL_ERUNTIME: return "runtime error";
+    case DTL_ELOGIC: return "the block count must not exceed 2^31";
+    case DTL_ENOMEM: return "out of device memory";
+    case DTL_EHIP: return "HIP runtime error";
+    case DTL_ENODEVICE: return "no usable GPU";
+  }
+  return "unknown status";
+}
+
+int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k) {
+  if (z > 1) s = w;
+  return bbf_find(word_bits, w, s, z > 1 ? z : 1, k) != nullptr;
+}
+
+int dtl_bbf_config_count(void) { return static_cast<int>(bbf_registry().size()); }
+
+int dtl_bbf_desc_init(dtl_bbf_desc* d, uint32_t word_bits, uint64_t m, uint32_t k, uint32_t w,
+                      uint32_t s, uint32_t z) {
+  if (!d) return DTL_EINVAL;
+  std::memset(d, 0, sizeof(*d));
+  if (word_bits != 32 && word_bits != 64) return DTL_ERUNTIME;  // filter.hpp:64
+  const uint32_t word_bytes = word_bits / 8;
+  uint32_t s_eff = 0, k_eff = 0;
+  if (z <= 1) {
+    // blocked_bloomfilter.hpp:169-229: unsupported values throw, others are
+    // clamped and the mismatch is reported after construction.
+    if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return DTL_EINVAL;
+    if (s != 1 && s != 2 && s != 4 && s != 8 && s != 16) return DTL_EINVAL;
+    s_eff = (s == 1) ? 1u : std::min(w * word_bytes, s);
+    if (k < 1 || k > 16) return DTL_EINVAL;
+    if (k == 1) k_eff = std::max(1u, s_eff);
+    else if (k == 16) k_eff = std::max(16u, s_eff);
+    else k_eff = std::max((k % s_eff == 0) ? k : 1u, s_eff);
+    z = 1;
+  } else {
+    // zoned_blocked_bloomfilter.hpp:102-215
+    if (w <= z) return DTL_EINVAL;
+    if (w != 4 && w != 8 && w != 16) return DTL_EINVAL;
+    if (z != 2 && z != 4 && z != 8) return DTL_EINVAL;
+    z = std::min(w / 2, z);
+    if (k < 2 || k > 16 || (k & 1u)) return DTL_EINVAL;
+    k_eff = (k == 16) ? std::max(16u, z) : std::max((k % z == 0) ? k : 2u, z);
+    s = s_eff = w;
+  }
+  Addressing a;
+  const int rc = make_addressing(m, static_cast<uint64_t>(word_bits) * w, &a);
+  if (rc != DTL_OK) return rc;
+  if (k_eff != k || s_eff != s) return DTL_EINVAL;
+  d->word_bits = word_bits;
+  d->w = w;
+  d->s = s_eff;
+  d->z = z;
+  d->k = k_eff;
+  d->addr_mode = a.mode;
+  d->block_cnt = a.block_cnt;
+  d->block_shift = a.shift;
+  d->magic = a.magic;
+  d->magic_shift = a.mshift;
+  d->m = m;
+  d->m_actual = static_cast<uint64_t>(a.block_cnt) * w * word_bits;
+  d->word_cnt = static_cast<uint64_t>(a.block_cnt) * w;
+  d->size_bytes = d->m_actual / 8;
+  return DTL_OK;
+}
+
+int dtl_cf_desc_init(dtl_cf_desc* d, uint64_t m, uint32_t tag_bits, uint32_t tags_per_bucket) {
+  if (!d) return DTL_EINVAL;
+  std::memset(d, 0, sizeof(*d));
+  // cuckoofilter.hpp:184-206
+  if (tag_bits != 4 && tag_bits != 8 && tag_bits != 12 && tag_bits != 16 && tag_bits != 32)
+    return DTL_EINVAL;
+  if (tags_per_bucket != 1 && tags_per_bucket != 2 && tags_per_bucket != 4) return DTL_EINVAL;
+  const uint32_t bpb = (tag_bits * tags_per_bucket + 7) / 8;  // cuckoofilter_table.hpp:89
+  Addressing a;
+  const int rc = make_addressing(m, static_cast<uint64_t>(bpb) * 8, &a);
+  if (rc != DTL_OK) return rc;
+  d->tag_bits = tag_bits;
+  d->tags_per_bucket = tags_per_bucket;
+  d->bytes_per_bucket = bpb;
+  d->addr_mode = a.mode;
+  d->bucket_cnt = a.block_cnt;
+  d->bucket_shift = a.shift;
+  d->bucket_mask = a.mask;
+  d->magic = a.magic;
+  d->magic_shift = a.mshift;
+  d->m = m;
+  d->m_actual = static_cast<uint64_t>(bpb) * a.block_cnt * 8;
+  d->table_words = (static_cast<uint64_t>(bpb) * a.block_cnt + 3) / 4;  // cuckoofilter_table.hpp:245-249
+  d->victim_offset = d->table_words * 4;
+  d->buffer_bytes = d->victim_offset + 16;
+  return DTL_OK;
+}
+
+int bbf_build(const dtl_bbf_desc* d, void* d_words, const uint32_t* d_keys, uint64_t n,
+              dtl_stream_t stream) {
+  if (!desc_ok(d) || !d_words || (n && !d_keys)) return DTL_EINVAL;
+  if (!aligned_to(d_words, std::min(16u, block_bytes(d)))) return DTL_EINVAL;
+  const BbfEntry* e = bbf_find(d->word_bits, d->w, d->s, d->z, d->k);
+  if (!e) return DTL_EINVAL;
+  if (n == 0) return DTL_OK;
+  DeviceInfo di;
+  int rc = current_device_info(&di, nullptr);
+  if (rc != DTL_OK) return rc;
+  BbfBuildLaunch L;
+  L.words = d_words;
+  L.keys = d_keys;
+  L.n = n;
+  fill_addr(d, &L.addr_magic, &L.addr_mshift, &L.addr_divisor, &L.addr_post);
+  L.cu_count = di.cu_count;
+  L.stream = static_cast<hipStream_t>(stream);
+  return hip_status(e->build(L));
+}
+
+int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
+                 uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count,
+                 const dtl_probe_params* params, dtl_stream_t stream) {
+  if (!desc_ok(d) || !d_words || !d_count || n > 0xFFFFFFFFull) return DTL_EINVAL;
+  if (n && (!d_keys || !d_pos)) return DTL_EINVAL;
+  if (!aligned_to(d_words, std::min(16u, block_bytes(d)))) return DTL_EINVAL;
+  const BbfEntry* e = bbf_find(d->word_bits, d->w, d->s, d->z, d->k);
+  if (!e) return DTL_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DeviceInfo di;
+  int dev = 0;
+  int rc = current_device_info(&di, &dev);
+  if (rc != DTL_OK) return rc;
+  if (n == 0) return hip_status(write_zero_count(d_count, st));
+  const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
+  const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
+  const size_t cnt_bytes = static_cast<size_t>(ntiles) * 4;
+  void* scratch = nullptr;
+  rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
+  if (rc != DTL_OK) return rc;
+  BbfProbeLaunch L;
+  L.words = d_words;
+  L.keys = d_keys;
+  L.n = static_cast<uint32_t>(n);
+  fill_addr(d, &L.addr_magic, &L.addr_mshift, &L.addr_divisor, &L.addr_post);
+  L.bits = static_cast<uint64_t*>(scratch);
+  L.tile_cnt = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + bits_bytes);
+  L.ntiles = ntiles;
+  L.filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d->size_bytes, 0xFFFFFFFFull));
+  uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
+  if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
+  const bool force_global = params && (params->flags & DTL_PROBE_FORCE_GLOBAL);
+  L.use_lds = !force_global && d->size_bytes <= lds_max;
+  uint32_t threads = (params && params->block_threads) ? params->block_threads : 0;
+  if (threads == 0) threads = d->size_bytes > 40 * 1024 ? 1024 : 256;
+  L.lds_threads = threads >= 1024 ? 1024 : 256;
+  L.lds_grid = params ? params->grid_blocks : 0;
+  L.cu_count = di.cu_count;
+  L.stream = st;
+  if (params && params->ev_start &&
+      hipEventRecord(static_cast<hipEvent_t>(params->ev_start), st) != hipSuccess)
+    return DTL_EHIP;
+  hipError_t he = e->probe(L);
+  if (he != hipSuccess) return DTL_EHIP;
+  if (params && params->ev_stop &&
+      hipEventRecord(static_cast<hipEvent_t>(params->ev_stop), st) != hipSuccess)
+    return DTL_EHIP;
+  return hip_status(compact_single(L.bits, L.tile_cnt, ntiles, L.n, match_offset, d_pos, d_count, st));
+}
+
+int bbf_probe(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
+              uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream) {
+  return bbf_probe_ex(d, d_words, d_keys, n, match_offset, d_pos, d_count, nullptr, stream);
+}
+
+int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
+             uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream) {
+  if (!d || !d_buf || !d_count || n > 0xFFFFFFFFull || d->bucket_cnt < 2) return DTL_EINVAL;
+  if (n && (!d_keys || !d_pos)) return DTL_EINVAL;
+  if (!aligned_to(d_buf, 16)) return DTL_EINVAL;
+  const CfEntry* e = cf_find(d->tag_bits, d->tags_per_bucket);
+  if (!e) return DTL_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DeviceInfo di;
+  int dev = 0;
+  int rc = current_device_info(&di, &dev);
+  if (rc != DTL_OK) return rc;
+  if (n == 0) return hip_status(write_zero_count(d_count, st));
+  const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
+  const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
+  const size_t cnt_bytes = static_cast<size_t>(ntiles) * 4;
+  void* scratch = nullptr;
+  rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
+  if (rc != DTL_OK) return rc;
+  CfProbeLaunch L;
+  L.buf = d_buf;
+  L.keys = d_keys;
+  L.n = static_cast<uint32_t>(n);
+  L.addr_mode = d->addr_mode;
+  L.shift = d->bucket_shift;
+  L.bucket_mask = d->bucket_mask;
+  L.magic = d->magic;
+  L.mshift = d->magic_shift;
+  L.bucket_cnt = d->bucket_cnt;
+  L.victim_offset = d->victim_offset;
+  L.bits = static_cast<uint64_t*>(scratch);
+  L.tile_cnt = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + bits_bytes);
+  L.ntiles = ntiles;
+  L.stream = st;
+  if (e->probe(L) != hipSuccess) return DTL_EHIP;
+  return hip_status(compact_single(L.bits, L.tile_cnt, ntiles, L.n, match_offset, d_pos, d_count, st));
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// dtl_compact.hip -- order-preserving compaction of the match bitmap.
+//
+// Phase 2 (scan): one workgroup per task scans the task's tile counts in
+// place into exclusive offsets and writes the task's match count.
+// Phase 3 (expand): one workgroup per tile; 256 threads each own 8 keys (one
+// byte of the bitmap), a workgroup exclusive scan of the byte popcounts gives
+// every match its rank, and positions i + match_offset are written in key
+// order: exactly the ascending selection vector of the reference
+// (blocked_bloomfilter_batch_dispatch.hpp:47-67).
+#include <hip/hip_runtime.h>
+
+#include "dtl_internal.hpp"
+#include "dtl_kernels.hpp"
+
+namespace dtlh {
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= static_cast<uint32_t>(d)) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over the workgroup in thread order; `total` receives the sum.
+template <int BT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* sh) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) sh[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t x = lane < static_cast<uint32_t>(BT / 64) ? sh[lane] : 0u;
+    x = wave_incl_scan(x);
+    if (lane < static_cast<uint32_t>(BT / 64)) sh[lane] = x;
+  }
+  __syncthreads();
+  const uint32_t before = wave ? sh[wave - 1] : 0u;
+  total = sh[BT / 64 - 1];
+  __syncthreads();
+  return before + incl - v;
+}
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+
+__device__ void scan_range(uint32_t* cnt, uint32_t begin, uint32_t count, uint64_t* d_count,
+                           uint32_t* sh) {
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < count; c0 += kScanThreads * kScanItems) {
+    uint32_t v[kScanItems];
+    uint32_t local = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t idx = c0 + threadIdx.x * kScanItems + i;
+      v[i] = idx < count ? cnt[begin + idx] : 0u;
+      local += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = carry + block_excl_scan<kScanThreads>(local, tot, sh);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t idx = c0 + threadIdx.x * kScanItems + i;
+      if (idx < count) cnt[begin + idx] = run;
+      run += v[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *d_count = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(uint32_t* cnt, uint32_t ntiles,
+                                                              uint64_t* d_count) {
+  __shared__ uint32_t sh[kScanThreads / 64];
+  scan_range(cnt, 0, ntiles, d_count, sh);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tasks(uint32_t* cnt,
+                                                             const BatchTaskDev* tasks) {
+  __shared__ uint32_t sh[kScanThreads / 64];
+  const BatchTaskDev& t = tasks[blockIdx.x];
+  scan_range(cnt, t.tile_begin, t.tile_count, t.count, sh);
+}
+
+constexpr int kExpandThreads = 256;
+static_assert(kExpandThreads * 8 == kTileKeys, "one bitmap byte per thread");
+
+__device__ __forceinline__ void expand_tile(const uint64_t* __restrict__ bits, uint32_t gtile,
+                                            uint32_t ltile, uint32_t offset, uint32_t match_offset,
+                                            uint32_t* __restrict__ pos, uint32_t* sh) {
+  const uint32_t t = threadIdx.x;
+  const uint64_t word = bits[static_cast<uint64_t>(gtile) * kTileWords + (t >> 3)];
+  uint32_t b8 = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
+  uint32_t tot;
+  uint32_t out = offset + block_excl_scan<kExpandThreads>(__popc(b8), tot, sh);
+  const uint32_t kb = ltile * kTileKeys + (t >> 3) * 64u + (t & 7u) * 8u;
+  while (b8) {
+    const uint32_t j = __ffs(b8) - 1u;
+    pos[out++] = kb + j + match_offset;
+    b8 &= b8 - 1u;
+  }
+}
+
+__global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
+                                                                  const uint32_t* __restrict__ offs,
+                                                                  uint32_t match_offset,
+                                                                  uint32_t* __restrict__ pos) {
+  __shared__ uint32_t sh[kExpandThreads / 64];
+  expand_tile(bits, blockIdx.x, blockIdx.x, offs[blockIdx.x], match_offset, pos, sh);
+}
+
+__global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
+                                                                 const uint32_t* __restrict__ offs,
+                                                                 const BatchTaskDev* __restrict__ tasks,
+                                                                 const uint32_t* __restrict__ tile_task) {
+  __shared__ uint32_t sh[kExpandThreads / 64];
+  const uint32_t g = blockIdx.x;
+  const BatchTaskDev& t = tasks[tile_task[g]];
+  expand_tile(bits, g, g - t.tile_begin, offs[g], t.match_offset, t.pos, sh);
+}
+
+__global__ void k_zero_count(uint64_t* d_count) { *d_count = 0; }
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+
+hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t ntiles, uint32_t n,
+                          uint32_t match_offset, uint32_t* pos, uint64_t* d_count,
+                          hipStream_t stream) {
+  (void)n;
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, stream, tile_cnt, ntiles,
+                     d_count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_expand_single, dim3(ntiles), dim3(kExpandThreads), 0, stream, bits,
+                     tile_cnt, match_offset, pos);
+  return hipGetLastError();
+}
+
+hipError_t compact_tasks(const uint64_t* bits, uint32_t* tile_cnt, const BatchTaskDev* tasks,
+                         const uint32_t* tile_task, uint32_t n_tasks, uint32_t total_tiles,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(k_scan_tasks, dim3(n_tasks), dim3(kScanThreads), 0, stream, tile_cnt, tasks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || total_tiles == 0) return e;
+  hipLaunchKernelGGL(k_expand_tasks, dim3(total_tiles), dim3(kExpandThreads), 0, stream, bits,
+                     tile_cnt, tasks, tile_task);
+  return hipGetLastError();
+}
+
+hipError_t write_zero_count(uint64_t* d_count, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, stream, d_count);
+  return hipGetLastError();
+}
+
+hipError_t zero_u32(uint32_t* p, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, p);
+  return hipGetLastError();
+}
+
+}  // namespace dtlh

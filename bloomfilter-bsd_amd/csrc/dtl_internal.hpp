@@ -1,0 +1,135 @@
+// dtl_internal.hpp -- host-side plumbing shared by the libdtl_hip sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "dtl_sched.hpp"
+
+namespace dtlh {
+
+constexpr uint32_t kTileKeys = 2048;             // keys per tile of every probe kernel
+constexpr uint32_t kTileWords = kTileKeys / 64;  // 64-bit match-bitmap words per tile
+
+// Arguments of one blocked-Bloom probe launch (phase 1: bitmap + tile counts).
+struct BbfProbeLaunch {
+  const void* words;
+  const uint32_t* keys;
+  uint32_t n;
+  uint32_t addr_magic, addr_mshift, addr_divisor, addr_post;
+  uint64_t* bits;
+  uint32_t* tile_cnt;
+  uint32_t ntiles;
+  uint32_t filter_bytes;
+  int use_lds;
+  uint32_t lds_threads;
+  uint32_t lds_grid;      // 0 = from occupancy
+  int cu_count;
+  hipStream_t stream;
+};
+
+struct BbfBuildLaunch {
+  void* words;
+  const uint32_t* keys;
+  uint64_t n;
+  uint32_t addr_magic, addr_mshift, addr_divisor, addr_post;
+  int cu_count;
+  hipStream_t stream;
+};
+
+// One batched probe task as the device sees it.
+struct BatchTaskDev {
+  const void* words;
+  const uint32_t* keys;
+  uint32_t* pos;
+  uint64_t* count;
+  uint32_t n;
+  uint32_t tile_begin;    // first global tile of the task
+  uint32_t tile_count;
+  uint32_t match_offset;
+  uint32_t addr_magic, addr_mshift, addr_divisor, addr_post;
+  uint32_t filter_bytes;  // staged in LDS when <= the call's LDS threshold
+};
+
+struct BbfBatchLaunch {
+  const BatchTaskDev* tasks;   // device array
+  const uint32_t* tile_task;   // device: global tile -> task index
+  uint32_t n_tasks;
+  uint32_t total_tiles;
+  uint64_t* bits;
+  uint32_t* tile_cnt;
+  uint32_t* work_counter;      // device, zero at launch start
+  uint32_t lds_max_bytes;
+  uint32_t threads;
+  uint32_t grid;
+  uint32_t chunk;              // tiles per grab
+  int cu_count;
+  hipStream_t stream;
+};
+
+struct BbfEntry {
+  uint32_t word_bits, w, s, z, k;
+  hipError_t (*probe)(const BbfProbeLaunch&);
+  hipError_t (*build)(const BbfBuildLaunch&);
+  hipError_t (*probe_batched)(const BbfBatchLaunch&);  // may be null
+};
+
+struct CfProbeLaunch {
+  const void* buf;
+  const uint32_t* keys;
+  uint32_t n;
+  uint32_t addr_mode, shift, bucket_mask, magic, mshift, bucket_cnt;
+  uint64_t victim_offset;
+  uint64_t* bits;
+  uint32_t* tile_cnt;
+  uint32_t ntiles;
+  hipStream_t stream;
+};
+
+struct CfEntry {
+  uint32_t tag_bits, tags_per_bucket;
+  hipError_t (*probe)(const CfProbeLaunch&);
+};
+
+// Registries (filled once, on first use).
+const std::vector<BbfEntry>& bbf_registry();
+const BbfEntry* bbf_find(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
+const CfEntry* cf_find(uint32_t tag_bits, uint32_t tags_per_bucket);
+
+// Per-(word type, w) registration functions, one translation unit each.
+#define DTL_DECLARE_REG(name) void name(std::vector<BbfEntry>& out);
+DTL_DECLARE_REG(dtl_reg_u32_w1)
+DTL_DECLARE_REG(dtl_reg_u32_w2)
+DTL_DECLARE_REG(dtl_reg_u32_w4)
+DTL_DECLARE_REG(dtl_reg_u32_w8)
+DTL_DECLARE_REG(dtl_reg_u32_w16)
+DTL_DECLARE_REG(dtl_reg_u64_w1)
+DTL_DECLARE_REG(dtl_reg_u64_w2)
+DTL_DECLARE_REG(dtl_reg_u64_w4)
+DTL_DECLARE_REG(dtl_reg_u64_w8)
+DTL_DECLARE_REG(dtl_reg_u64_w16)
+DTL_DECLARE_REG(dtl_reg_z32_w4)
+DTL_DECLARE_REG(dtl_reg_z32_w8)
+DTL_DECLARE_REG(dtl_reg_z32_w16)
+DTL_DECLARE_REG(dtl_reg_z64_w4)
+DTL_DECLARE_REG(dtl_reg_z64_w8)
+DTL_DECLARE_REG(dtl_reg_z64_w16)
+#undef DTL_DECLARE_REG
+
+// Compaction (dtl_compact.hip): exclusive scan of the tile counts (in place)
+// and expansion of the bitmap into positions.
+hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t ntiles, uint32_t n,
+                          uint32_t match_offset, uint32_t* pos, uint64_t* d_count,
+                          hipStream_t stream);
+hipError_t compact_tasks(const uint64_t* bits, uint32_t* tile_cnt, const BatchTaskDev* tasks,
+                         const uint32_t* tile_task, uint32_t n_tasks, uint32_t total_tiles,
+                         hipStream_t stream);
+hipError_t write_zero_count(uint64_t* d_count, hipStream_t stream);
+hipError_t zero_u32(uint32_t* p, hipStream_t stream);
+
+// Device info of the current device (CU count, LDS bytes per workgroup).
+int current_device(int* dev, int* cu_count, uint32_t* max_lds);
+
+}  // namespace dtlh

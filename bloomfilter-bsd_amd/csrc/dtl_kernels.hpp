@@ -1,0 +1,285 @@
+// dtl_kernels.hpp -- gfx950 kernels of the filter probe path (templates).
+//
+// Work decomposition (all probe kernels):
+//   * keys are cut into tiles of kTileKeys consecutive keys; a workgroup
+//     handles whole tiles, one key per lane per round, rounds of one key
+//     per thread so each wave reads 64 consecutive keys (256 B, coalesced);
+//   * each wave turns its 64 hit flags into one 64-bit ballot word of the
+//     match bitmap (bit i <-> key i) and the workgroup writes one match count
+//     per tile;
+//   * compaction (dtl_compact.hip) scans the tile counts and expands the
+//     bitmap into ascending positions, so the list comes out in the
+//     reference's order without atomics.
+// Filter access:
+//   * "global" kernels load the key's whole block (<= 128 B) into VGPRs with
+//     16-byte loads; the filter stays in HBM / Infinity Cache / L2;
+//   * "lds" kernels first stage the whole filter in LDS (filters up to the
+//     workgroup LDS limit, 160 KiB on gfx950) and then loop over tiles
+//     (persistent grid), reading only the words a key needs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "dtl_internal.hpp"
+#include "dtl_sched.hpp"
+
+namespace dtlh {
+
+
+// Block addressing (block_addressing_logic.hpp:104-110 MAGIC, :205-210 POW2),
+// evaluated branch-free as ((h - (mulhi(h, magic) >> mshift) * divisor) >> post):
+//   POW2:  magic = divisor = 0, post = 32 - log2(block_cnt)  ->  h >> post
+//   MAGIC: post = 0                                          ->  h mod divisor
+struct AddrArgs {
+  uint32_t magic;
+  uint32_t mshift;
+  uint32_t divisor;
+  uint32_t post;
+};
+
+__device__ __forceinline__ uint32_t block_of(uint32_t h0, const AddrArgs& a) {
+  const uint32_t q = __umulhi(h0, a.magic) >> a.mshift;
+  return (h0 - q * a.divisor) >> a.post;
+}
+
+__device__ __forceinline__ uint32_t hslot(uint32_t key, int src) {
+  return src == 0 ? 0u : key * kMul.v[src];
+}
+
+__device__ __forceinline__ uint32_t chunk_val(uint32_t key, Chunk c) {
+  return ((hslot(key, c.src) >> c.shift) & ((1u << c.nbits) - 1u)) + c.base;
+}
+
+template <class W>
+__device__ __forceinline__ W one_at(uint32_t bit) {
+  return static_cast<W>(1) << bit;
+}
+
+// Per-group search masks of a key.
+template <class C>
+__device__ __forceinline__ void key_masks(uint32_t key, typename C::word_t (&mask)[C::sched.ngroups]) {
+  using W = typename C::word_t;
+  static_for<C::sched.ngroups>([&](auto g) { mask[g] = 0; });
+  static_for<C::sched.nbits>([&](auto b) {
+    constexpr Chunk c = C::sched.bbit[b];
+    constexpr int g = C::sched.bgroup[b];
+    mask[g] |= one_at<W>(chunk_val(key, c));
+  });
+}
+
+template <int BASE, int SPAN, class W, int N>
+__device__ __forceinline__ W select_word(const W (&blk)[N], uint32_t off) {
+  W r = blk[BASE];
+  static_for<SPAN - 1>([&](auto j) {
+    constexpr int J = decltype(j)::value + 1;
+    r = (off == static_cast<uint32_t>(J)) ? blk[BASE + J] : r;
+  });
+  return r;
+}
+
+// Loads a whole block (w words) into registers with the widest aligned loads.
+template <class C>
+__device__ __forceinline__ void load_block(const typename C::word_t* __restrict__ p,
+                                           typename C::word_t (&blk)[C::w]) {
+  using W = typename C::word_t;
+  constexpr int BYTES = C::w * sizeof(W);
+  if constexpr (BYTES >= 16) {
+    uint4 v[BYTES / 16];
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    static_for<BYTES / 16>([&](auto i) { v[i] = q[i]; });
+    __builtin_memcpy(blk, v, BYTES);
+  } else if constexpr (BYTES == 8) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    __builtin_memcpy(blk, &v, 8);
+  } else {
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+    __builtin_memcpy(blk, &v, 4);
+  }
+}
+
+// Hit test against a block held in registers.
+template <class C>
+__device__ __forceinline__ bool test_regs(uint32_t key, const typename C::word_t (&blk)[C::w]) {
+  using W = typename C::word_t;
+  W mask[C::sched.ngroups];
+  key_masks<C>(key, mask);
+  bool ok = true;
+  static_for<C::sched.ngroups>([&](auto g) {
+    constexpr Chunk d = C::sched.gdyn[g];
+    constexpr int base = C::sched.gword[g];
+    W word;
+    if constexpr (d.nbits == 0) {
+      word = blk[base];
+    } else {
+      word = select_word<base, (1 << d.nbits)>(blk, chunk_val(key, d));
+    }
+    ok &= (word & mask[g]) == mask[g];
+  });
+  return ok;
+}
+
+// Hit test reading only the needed words (LDS or global pointer to the block).
+template <class C, class P>
+__device__ __forceinline__ bool test_ptr(uint32_t key, P blk) {
+  using W = typename C::word_t;
+  W mask[C::sched.ngroups];
+  key_masks<C>(key, mask);
+  bool ok = true;
+  static_for<C::sched.ngroups>([&](auto g) {
+    constexpr Chunk d = C::sched.gdyn[g];
+    constexpr int base = C::sched.gword[g];
+    uint32_t wi = base;
+    if constexpr (d.nbits != 0) wi += chunk_val(key, d);
+    const W word = blk[wi];
+    ok &= (word & mask[g]) == mask[g];
+  });
+  return ok;
+}
+
+__device__ __forceinline__ uint32_t load_key(const uint32_t* __restrict__ keys, uint64_t i) {
+  return __builtin_nontemporal_load(keys + i);
+}
+
+__device__ __forceinline__ void store_bits(uint64_t* __restrict__ bits, uint64_t i, uint64_t v) {
+  __builtin_nontemporal_store(v, bits + i);
+}
+
+// Sums a per-wave uniform value over the workgroup; result valid in thread 0.
+template <int BT>
+__device__ __forceinline__ uint32_t block_sum_waves(uint32_t v, uint32_t* red) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  uint32_t s = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BT / 64; ++i) s += red[i];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Bloom filter probe, filter read from global memory.
+// grid = number of tiles, block = 256 threads (8 rounds x 4 waves = 32 words).
+// ---------------------------------------------------------------------------
+constexpr int kGlobalThreads = 256;
+constexpr int kGlobalUnroll = 4;
+
+template <class C>
+__global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
+    const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
+    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt) {
+  using W = typename C::word_t;
+  constexpr int BT = kGlobalThreads;
+  constexpr int ROUNDS = kTileKeys / BT;
+  constexpr int U = kGlobalUnroll;
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kTileKeys;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r0 = 0; r0 < ROUNDS; r0 += U) {
+    uint32_t key[U];
+    bool valid[U];
+    W blk[U][C::w];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = tile_base + static_cast<uint64_t>(r0 + u) * BT + tid;
+      valid[u] = i < n;
+      key[u] = load_key(keys, valid[u] ? i : n - 1);  // n >= 1: clamp, no branch
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t b = block_of(key[u] * kMul.v[0], addr);
+      load_block<C>(filter + static_cast<uint64_t>(b) * C::w, blk[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool hit = test_regs<C>(key[u], blk[u]) & valid[u];
+      const uint64_t m = __ballot(hit);
+      if (lane == 0) store_bits(bits, (tile_base + static_cast<uint64_t>(r0 + u) * BT) / 64 + wave, m);
+      cnt += __popcll(m);
+    }
+  }
+  const uint32_t total = block_sum_waves<BT>(cnt, red);
+  if (tid == 0) tile_cnt[blockIdx.x] = total;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Bloom filter probe, filter staged in LDS (persistent grid).
+// ---------------------------------------------------------------------------
+template <class C, int BT>
+__global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* __restrict__ filter,
+                                                      uint32_t filter_bytes,
+                                                      const uint32_t* __restrict__ keys, uint32_t n,
+                                                      AddrArgs addr, uint64_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ tile_cnt,
+                                                      uint32_t ntiles) {
+  using W = typename C::word_t;
+  constexpr int ROUNDS = kTileKeys / BT;
+  extern __shared__ uint4 lds_filter[];
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  {
+    const uint32_t n16 = filter_bytes / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(filter);
+    for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
+    uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds_filter);
+    const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
+    for (uint32_t i = n16 * 4 + tid; i < filter_bytes / 4; i += BT) dst32[i] = src32[i];
+  }
+  __syncthreads();
+  const W* lds = reinterpret_cast<const W*>(lds_filter);
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t tile_base = static_cast<uint64_t>(t) * kTileKeys;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+      const uint64_t i = tile_base + static_cast<uint64_t>(r) * BT + tid;
+      const bool valid = i < n;
+      const uint32_t key = load_key(keys, valid ? i : n - 1);
+      const uint32_t b = block_of(key * kMul.v[0], addr);
+      const bool hit = test_ptr<C>(key, lds + static_cast<uint32_t>(b * C::w)) & valid;
+      const uint64_t m = __ballot(hit);
+      if (lane == 0) store_bits(bits, (tile_base + static_cast<uint64_t>(r) * BT) / 64 + wave, m);
+      cnt += __popcll(m);
+    }
+    const uint32_t total = block_sum_waves<BT>(cnt, red);
+    if (tid == 0) tile_cnt[t] = total;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Bloom filter build: one key per thread, one atomicOr per group.
+// ---------------------------------------------------------------------------
+template <class W>
+__device__ __forceinline__ void atomic_or_word(W* p, W v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class C>
+__global__ __launch_bounds__(256) void k_bbf_build(typename C::word_t* __restrict__ filter,
+                                                   const uint32_t* __restrict__ keys, uint64_t n,
+                                                   AddrArgs addr) {
+  using W = typename C::word_t;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t key = load_key(keys, i);
+    const uint32_t b = block_of(key * kMul.v[0], addr);
+    W* blk = filter + static_cast<uint64_t>(b) * C::w;
+    W mask[C::sched.ngroups];
+    key_masks<C>(key, mask);
+    static_for<C::sched.ngroups>([&](auto g) {
+      constexpr Chunk d = C::sched.gdyn[g];
+      uint32_t wi = C::sched.gword[g];
+      if constexpr (d.nbits != 0) wi += chunk_val(key, d);
+      atomic_or_word<W>(blk + wi, mask[g]);
+    });
+  }
+}
+
+}  // namespace dtlh

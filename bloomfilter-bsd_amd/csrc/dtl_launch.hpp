@@ -1,0 +1,118 @@
+// dtl_launch.hpp -- host launchers of the templated kernels and the
+// compile-time enumeration of every configuration the reference dispatches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dtl_batched.hpp"
+#include "dtl_internal.hpp"
+#include "dtl_kernels.hpp"
+
+namespace dtlh {
+
+inline AddrArgs make_addr(uint32_t magic, uint32_t mshift, uint32_t divisor, uint32_t post) {
+  AddrArgs a;
+  a.magic = magic;
+  a.mshift = mshift;
+  a.divisor = divisor;
+  a.post = post;
+  return a;
+}
+
+template <class C, int BT>
+hipError_t launch_probe_lds(const BbfProbeLaunch& L) {
+  using W = typename C::word_t;
+  auto fn = k_bbf_probe_lds<C, BT>;
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)attr;
+  const uint32_t lds_bytes = (L.filter_bytes + 15u) & ~15u;
+  uint32_t grid = L.lds_grid;
+  if (grid == 0) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BT, lds_bytes);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    grid = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(std::max(L.cu_count, 1));
+  }
+  grid = std::max(1u, std::min(grid, L.ntiles));
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(BT), lds_bytes, L.stream,
+                     static_cast<const W*>(L.words), L.filter_bytes, L.keys, L.n,
+                     make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post),
+                     L.bits, L.tile_cnt, L.ntiles);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_probe(const BbfProbeLaunch& L) {
+  using W = typename C::word_t;
+  if (L.use_lds) {
+    switch (L.lds_threads) {
+      case 1024: return launch_probe_lds<C, 1024>(L);
+      default: return launch_probe_lds<C, 256>(L);
+    }
+  }
+  hipLaunchKernelGGL(k_bbf_probe_global<C>, dim3(L.ntiles), dim3(kGlobalThreads), 0, L.stream,
+                     static_cast<const W*>(L.words), L.keys, L.n,
+                     make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post),
+                     L.bits, L.tile_cnt);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_build(const BbfBuildLaunch& L) {
+  using W = typename C::word_t;
+  const uint64_t want = (L.n + 255) / 256;
+  const uint64_t cap = static_cast<uint64_t>(std::max(L.cu_count, 1)) * 64;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+  hipLaunchKernelGGL(k_bbf_build<C>, dim3(grid), dim3(256), 0, L.stream, static_cast<W*>(L.words),
+                     L.keys, L.n,
+                     make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post));
+  return hipGetLastError();
+}
+
+template <class C>
+BbfEntry make_entry() {
+  BbfEntry e;
+  e.word_bits = C::word_bits;
+  e.w = C::w;
+  e.s = C::s;
+  e.z = C::z;
+  e.k = C::k;
+  e.probe = &launch_probe<C>;
+  e.build = &launch_build<C>;
+  e.probe_batched = &launch_probe_batched<C>;
+  return e;
+}
+
+// Every (s, k) the reference's sectorized filter accepts for this word type and w.
+template <class W, uint32_t Wc>
+void register_sectorized(std::vector<BbfEntry>& out) {
+  static_for<5>([&](auto si) {
+    constexpr uint32_t S = 1u << decltype(si)::value;
+    static_for<16>([&](auto ki) {
+      constexpr uint32_t K = decltype(ki)::value + 1;
+      if constexpr (valid_sgew_sltw(sizeof(W) * 8, Wc, S, K)) {
+        out.push_back(make_entry<BbfCfg<W, Wc, S, 1, K>>());
+      }
+    });
+  });
+}
+
+// Every (z, k) the reference's zoned filter accepts for this word type and w.
+template <class W, uint32_t Wc>
+void register_zoned(std::vector<BbfEntry>& out) {
+  static_for<3>([&](auto zi) {
+    constexpr uint32_t Z = 2u << decltype(zi)::value;
+    static_for<8>([&](auto ki) {
+      constexpr uint32_t K = 2 * (decltype(ki)::value + 1);
+      if constexpr (valid_zoned(sizeof(W) * 8, Wc, Z, K)) {
+        out.push_back(make_entry<BbfCfg<W, Wc, Wc, Z, K>>());
+      }
+    });
+  });
+}
+
+}  // namespace dtlh

@@ -1,0 +1,6 @@
+// Kernels of the sectorized blocked Bloom filter, 64-bit words, w = 4.
+#include "../dtl_launch.hpp"
+
+namespace dtlh {
+void dtl_reg_u64_w4(std::vector<BbfEntry>& out) { register_sectorized<uint64_t, 4>(out); }
+}  // namespace dtlh

@@ -1,0 +1,6 @@
+// Kernels of the zoned blocked Bloom filter, 32-bit words, w = 8.
+#include "../dtl_launch.hpp"
+
+namespace dtlh {
+void dtl_reg_z32_w8(std::vector<BbfEntry>& out) { register_zoned<uint32_t, 8>(out); }
+}  // namespace dtlh

@@ -1,0 +1,240 @@
+"""ctypes binding of libdtl_hip (include/dtl_hip.h) for tests and benchmarks.
+
+Device memory and streams come from PyTorch (plumbing only); every filter
+operation runs in the library's gfx950 kernels.  Loading fails loudly when the
+library has not been built, and every call raises on a non-OK status: there
+is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdtl_hip.so")
+
+OK, EINVAL, ERUNTIME, ELOGIC, ENOMEM, EHIP, ENODEVICE = 0, -1, -2, -3, -4, -5, -6
+ADDR_POW2, ADDR_MAGIC = 0, 1
+PROBE_FORCE_GLOBAL = 1
+
+
+class BbfDesc(C.Structure):
+    _fields_ = [("word_bits", C.c_uint32), ("w", C.c_uint32), ("s", C.c_uint32), ("z", C.c_uint32),
+                ("k", C.c_uint32), ("addr_mode", C.c_uint32), ("block_cnt", C.c_uint32),
+                ("block_shift", C.c_uint32), ("magic", C.c_uint32), ("magic_shift", C.c_uint32),
+                ("m", C.c_uint64), ("m_actual", C.c_uint64), ("word_cnt", C.c_uint64),
+                ("size_bytes", C.c_uint64)]
+
+
+class CfDesc(C.Structure):
+    _fields_ = [("tag_bits", C.c_uint32), ("tags_per_bucket", C.c_uint32),
+                ("bytes_per_bucket", C.c_uint32), ("addr_mode", C.c_uint32),
+                ("bucket_cnt", C.c_uint32), ("bucket_shift", C.c_uint32),
+                ("bucket_mask", C.c_uint32), ("magic", C.c_uint32), ("magic_shift", C.c_uint32),
+                ("reserved", C.c_uint32), ("m", C.c_uint64), ("m_actual", C.c_uint64),
+                ("table_words", C.c_uint64), ("victim_offset", C.c_uint64),
+                ("buffer_bytes", C.c_uint64)]
+
+
+class ProbeParams(C.Structure):
+    _fields_ = [("lds_max_bytes", C.c_uint32), ("block_threads", C.c_uint32),
+                ("grid_blocks", C.c_uint32), ("chunk_tiles", C.c_uint32), ("flags", C.c_uint32),
+                ("ev_start", C.c_void_p), ("ev_stop", C.c_void_p)]
+
+
+class BbfTask(C.Structure):
+    _fields_ = [("desc", BbfDesc), ("d_words", C.c_void_p), ("d_keys", C.c_void_p),
+                ("n", C.c_uint64), ("match_offset", C.c_uint32), ("reserved", C.c_uint32),
+                ("d_pos", C.c_void_p), ("d_count", C.c_void_p)]
+
+
+class DtlError(RuntimeError):
+    def __init__(self, fn, status):
+        super().__init__("%s failed: %s (%d)" % (fn, status_string(status), status))
+        self.status = status
+
+
+EXPORTS = [
+    "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
+    "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
+    "bbf_probe_batched", "dtl_status_string", "dtl_bbf_supported", "dtl_bbf_config_count",
+    "dtl_version",
+]
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads the library.  torch is imported first so that the library binds to
+    the HIP runtime torch already uses (one runtime per process)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libdtl_hip.so is not built (%s); run __graft_entry__.build()" % path)
+    import torch  # noqa: F401  (binds the process to torch's HIP runtime)
+    L = C.CDLL(path)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    L.dtl_bbf_desc_init.argtypes = [C.POINTER(BbfDesc), u32, u64, u32, u32, u32, u32]
+    L.dtl_cf_desc_init.argtypes = [C.POINTER(CfDesc), u64, u32, u32]
+    L.bbf_build.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, vp]
+    L.bbf_probe.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, vp]
+    L.bbf_probe_ex.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
+    L.cf_probe.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, vp]
+    L.cf_build_host.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, C.POINTER(C.c_uint64)]
+    L.bbf_batch_plan_create.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), C.POINTER(vp)]
+    L.bbf_batch_plan_run.argtypes = [vp, vp]
+    L.bbf_batch_plan_destroy.argtypes = [vp]
+    L.bbf_batch_plan_destroy.restype = None
+    L.bbf_probe_batched.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), vp]
+    L.dtl_status_string.argtypes = [i32]
+    L.dtl_status_string.restype = C.c_char_p
+    L.dtl_bbf_supported.argtypes = [u32, u32, u32, u32, u32]
+    L.dtl_bbf_config_count.argtypes = []
+    L.dtl_version.argtypes = []
+    for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
+                 "cf_probe", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
+                 "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_version"):
+        getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def status_string(status):
+    return load().dtl_status_string(status).decode()
+
+
+def _check(fn, rc):
+    if rc != OK:
+        raise DtlError(fn, rc)
+
+
+def bbf_desc(word_bits, m, k, w=1, s=1, z=1):
+    d = BbfDesc()
+    _check("dtl_bbf_desc_init", load().dtl_bbf_desc_init(C.byref(d), word_bits, m, k, w, s, z))
+    return d
+
+
+def bbf_desc_rc(word_bits, m, k, w=1, s=1, z=1):
+    d = BbfDesc()
+    return load().dtl_bbf_desc_init(C.byref(d), word_bits, m, k, w, s, z)
+
+
+def cf_desc(m, tag_bits=16, tags_per_bucket=4):
+    d = CfDesc()
+    _check("dtl_cf_desc_init", load().dtl_cf_desc_init(C.byref(d), m, tag_bits, tags_per_bucket))
+    return d
+
+
+def _stream(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def new_filter(desc, device="cuda"):
+    """Zeroed device buffer for a blocked Bloom filter (uint8 tensor)."""
+    import torch
+    return torch.zeros(int(desc.size_bytes), dtype=torch.uint8, device=device)
+
+
+def bbf_build(desc, words, keys, stream=None):
+    _check("bbf_build", load().bbf_build(C.byref(desc), _ptr(words), _ptr(keys), keys.numel(),
+                                         _stream(stream)))
+
+
+def bbf_probe(desc, words, keys, match_offset=0, params=None, stream=None, out=None, count=None):
+    """Returns (positions tensor of capacity n, device count tensor)."""
+    import torch
+    n = keys.numel()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
+    if count is None:
+        count = torch.zeros(1, dtype=torch.int64, device=keys.device)
+    L = load()
+    if params is None:
+        rc = L.bbf_probe(C.byref(desc), _ptr(words), _ptr(keys), n, match_offset, _ptr(out),
+                         _ptr(count), _stream(stream))
+    else:
+        rc = L.bbf_probe_ex(C.byref(desc), _ptr(words), _ptr(keys), n, match_offset, _ptr(out),
+                            _ptr(count), C.byref(params), _stream(stream))
+    _check("bbf_probe", rc)
+    return out, count
+
+
+def positions(out, count):
+    """Host numpy copy (uint32) of the first `count` positions."""
+    import numpy as np
+    c = int(count.item())
+    return out[:c].cpu().numpy().view(np.uint32).copy()
+
+
+def cf_build_host(desc, keys_np, seed=0):
+    """Host build into a zeroed numpy buffer of desc.buffer_bytes (padded to 8)."""
+    import numpy as np
+    keys_np = np.ascontiguousarray(keys_np, dtype=np.uint32)
+    buf = np.zeros((desc.buffer_bytes + 7) // 8 * 8, dtype=np.uint8)
+    ins = C.c_uint64(0)
+    _check("cf_build_host", load().cf_build_host(C.byref(desc), C.c_void_p(buf.ctypes.data),
+                                                 C.c_void_p(keys_np.ctypes.data), len(keys_np),
+                                                 seed, C.byref(ins)))
+    return buf, ins.value
+
+
+def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None):
+    import torch
+    n = keys.numel()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
+    if count is None:
+        count = torch.zeros(1, dtype=torch.int64, device=keys.device)
+    _check("cf_probe", load().cf_probe(C.byref(desc), _ptr(buf), _ptr(keys), n, match_offset,
+                                       _ptr(out), _ptr(count), _stream(stream)))
+    return out, count
+
+
+class BatchPlan:
+    """A bbf batch plan over a list of (desc, words, keys, match_offset) tasks."""
+
+    def __init__(self, tasks, params=None):
+        import torch
+        self.L = load()
+        self.outs, self.counts = [], []
+        arr = (BbfTask * len(tasks))()
+        for i, (desc, words, keys, off) in enumerate(tasks):
+            out = torch.empty(max(keys.numel(), 1), dtype=torch.int32, device=keys.device)
+            cnt = torch.zeros(1, dtype=torch.int64, device=keys.device)
+            self.outs.append(out)
+            self.counts.append(cnt)
+            arr[i].desc = desc
+            arr[i].d_words = words.data_ptr()
+            arr[i].d_keys = keys.data_ptr()
+            arr[i].n = keys.numel()
+            arr[i].match_offset = off
+            arr[i].d_pos = out.data_ptr()
+            arr[i].d_count = cnt.data_ptr()
+        self._keep = tasks
+        self.plan = C.c_void_p()
+        pp = C.byref(params) if params is not None else None
+        _check("bbf_batch_plan_create",
+               self.L.bbf_batch_plan_create(arr, len(tasks), pp, C.byref(self.plan)))
+
+    def run(self, stream=None):
+        _check("bbf_batch_plan_run", self.L.bbf_batch_plan_run(self.plan, _stream(stream)))
+
+    def results(self):
+        return [positions(o, c) for o, c in zip(self.outs, self.counts)]
+
+    def close(self):
+        if self.plan:
+            self.L.bbf_batch_plan_destroy(self.plan)
+            self.plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

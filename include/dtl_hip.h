@@ -1,0 +1,188 @@
+/*
+ * dtl_hip.h -- C ABI of libdtl_hip, the gfx950 (MI355X) filter probe path.
+ *
+ * It replaces the probe/build entry points of the reference's filter classes
+ * (peterboncz/bloomfilter-bsd, "DTL"):
+ *
+ *   bbf_build    dtl::filter::filter_base::batch_insert      filter_base.hpp:25-26
+ *                (blocked_bloomfilter_logic::batch_insert     blocked_bloomfilter_logic.hpp:111-117)
+ *   bbf_probe    dtl::filter::filter_base::batch_contains    filter_base.hpp:41-44
+ *                (internal::dispatch<..>::batch_contains       blocked_bloomfilter_batch_dispatch.hpp:28-69)
+ *   cf_probe     dtl::cf::batch_contains                      cf.cpp:49-54
+ *                (cuckoofilter_logic::batch_contains           cuckoofilter_logic.hpp:390-479)
+ *   cf_build_host dtl::cf::batch_insert (host build)          cf.cpp:39-42
+ *                (cuckoofilter_logic::insert                   cuckoofilter_logic.hpp:291-337)
+ *   bbf_probe_batched  no single reference counterpart: many filter_base::
+ *                batch_contains calls (one per filter) in one launch.
+ *
+ * Conventions
+ *   - All buffers passed to bbf_* / cf_probe are device pointers (HBM) owned
+ *     by the caller; calls are ordered on `stream` (a hipStream_t; NULL is the
+ *     default stream) and return once the work is enqueued.
+ *   - Filter words are laid out exactly as the reference lays them out
+ *     (block-major, little-endian, no header), so a byte compare against a
+ *     reference-built filter is meaningful.
+ *   - Match positions: the first *d_count entries of d_pos receive the
+ *     ascending positions i + match_offset of every key i that may be in the
+ *     filter (the reference's selection vector).  d_pos must hold n entries.
+ *     *d_count is a device uint64_t written by the call.
+ *   - No exceptions cross the ABI: every function returns a dtl_status.
+ */
+#ifndef DTL_HIP_H
+#define DTL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* dtl_stream_t; /* hipStream_t */
+
+typedef enum dtl_status {
+  DTL_OK = 0,
+  DTL_EINVAL = -1,      /* std::invalid_argument in the reference */
+  DTL_ERUNTIME = -2,    /* std::runtime_error */
+  DTL_ELOGIC = -3,      /* std::logic_error (block count out of range) */
+  DTL_ENOMEM = -4,      /* device allocation failed */
+  DTL_EHIP = -5,        /* a HIP runtime call failed */
+  DTL_ENODEVICE = -6    /* no usable GPU */
+} dtl_status;
+
+enum { DTL_ADDR_POW2 = 0, DTL_ADDR_MAGIC = 1 };
+
+/* Blocked Bloom filter descriptor, computed on the host by dtl_bbf_desc_init
+ * from the same parameters the reference's constructors take
+ * (blocked_bloomfilter.hpp:96-116; zoned_blocked_bloomfilter.hpp:98-120). */
+typedef struct dtl_bbf_desc {
+  uint32_t word_bits;   /* 32 or 64 */
+  uint32_t w;           /* words per block */
+  uint32_t s;           /* sectors per block (zoned: == w) */
+  uint32_t z;           /* zones per block (1: not zoned) */
+  uint32_t k;           /* bits set per key */
+  uint32_t addr_mode;   /* DTL_ADDR_POW2 when the requested m is a power of two */
+  uint32_t block_cnt;
+  uint32_t block_shift; /* POW2: 32 - log2(block_cnt) */
+  uint32_t magic;       /* MAGIC: block = h0 - ((mulhi(h0, magic) >> magic_shift) * block_cnt) */
+  uint32_t magic_shift;
+  uint64_t m;           /* requested length in bits */
+  uint64_t m_actual;    /* block_cnt * w * word_bits */
+  uint64_t word_cnt;    /* number of words of word_bits bits */
+  uint64_t size_bytes;  /* m_actual / 8: bytes the caller allocates (zeroed) */
+} dtl_bbf_desc;
+
+/* Cuckoo filter descriptor (cuckoofilter.hpp:118-127, cuckoofilter_logic.hpp:153-158).
+ * Device buffer layout: packed buckets (table_words u32 words), then the
+ * victim cache {u32 index; u32 tag; u8 used} at victim_offset, then padding:
+ * allocate buffer_bytes, zeroed.  (The reference places the victim cache
+ * after the table but sizes its buffers without it.) */
+typedef struct dtl_cf_desc {
+  uint32_t tag_bits;         /* 4, 8, 12, 16 or 32 */
+  uint32_t tags_per_bucket;  /* 1, 2 or 4 */
+  uint32_t bytes_per_bucket;
+  uint32_t addr_mode;
+  uint32_t bucket_cnt;
+  uint32_t bucket_shift;     /* POW2: 32 - log2(bucket_cnt) */
+  uint32_t bucket_mask;      /* POW2: bucket_cnt - 1 */
+  uint32_t magic;
+  uint32_t magic_shift;
+  uint32_t reserved;
+  uint64_t m;
+  uint64_t m_actual;         /* bytes_per_bucket * bucket_cnt * 8 */
+  uint64_t table_words;
+  uint64_t victim_offset;
+  uint64_t buffer_bytes;
+} dtl_cf_desc;
+
+/* Tuning knobs of one probe launch (0 = library default). */
+typedef struct dtl_probe_params {
+  uint32_t lds_max_bytes;   /* filters up to this size are staged in LDS (default: device LDS limit) */
+  uint32_t block_threads;   /* threads per workgroup of the persistent kernels (256 or 1024) */
+  uint32_t grid_blocks;     /* workgroups of the persistent kernels (default: CUs x occupancy) */
+  uint32_t chunk_tiles;     /* batched: tiles of 2048 keys a workgroup takes per grab (default 1) */
+  uint32_t flags;           /* DTL_PROBE_FORCE_GLOBAL: never stage in LDS */
+  void* ev_start;           /* optional hipEvent_t recorded right before the probe kernel */
+  void* ev_stop;            /* optional hipEvent_t recorded right after it (before compaction) */
+} dtl_probe_params;
+
+enum { DTL_PROBE_FORCE_GLOBAL = 1u };
+
+/* ---- descriptors -------------------------------------------------------- */
+/* Validates (word_bits, k, w, s, z) like the reference's runtime dispatch and
+ * sizes the filter for m requested bits; z = 1 selects the sectorized filter
+ * (dtl::blocked_bloomfilter), z > 1 the zoned one (dtl::zoned_blocked_bloomfilter,
+ * s is then ignored). */
+int dtl_bbf_desc_init(dtl_bbf_desc* d, uint32_t word_bits, uint64_t m, uint32_t k, uint32_t w,
+                      uint32_t s, uint32_t z);
+int dtl_cf_desc_init(dtl_cf_desc* d, uint64_t m, uint32_t tag_bits, uint32_t tags_per_bucket);
+
+/* ---- blocked Bloom filter ---------------------------------------------- */
+/* Inserts n keys with atomicOr (order-independent, so the words equal the
+ * reference's sequential insert bit for bit). */
+int bbf_build(const dtl_bbf_desc* d, void* d_words, const uint32_t* d_keys, uint64_t n,
+              dtl_stream_t stream);
+
+/* Probes n keys (n < 2^32) and writes the ascending match positions. */
+int bbf_probe(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
+              uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
+
+/* bbf_probe with explicit launch parameters (NULL = defaults). */
+int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
+                 uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count,
+                 const dtl_probe_params* params, dtl_stream_t stream);
+
+/* ---- cuckoo filter ----------------------------------------------------- */
+int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
+             uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
+
+/* Host build: inserts n keys into a zeroed host buffer of buffer_bytes,
+ * kicking victims with mt19937(seed) (the reference seeds a per-thread
+ * mt19937 from std::random_device, thread.hpp:71-79).  *inserted receives the
+ * number of keys for which insert returned true; returns DTL_OK even when
+ * the table overflowed (check *inserted). */
+int cf_build_host(const dtl_cf_desc* d, void* h_buf, const uint32_t* h_keys, uint64_t n,
+                  uint32_t seed, uint64_t* inserted);
+
+/* ---- batched probe ------------------------------------------------------ */
+/* One probe task: a filter with its own probe batch and output list. */
+typedef struct dtl_bbf_task {
+  dtl_bbf_desc desc;        /* (word_bits, w, s, z, k) must match across a call */
+  const void* d_words;
+  const uint32_t* d_keys;
+  uint64_t n;
+  uint32_t match_offset;
+  uint32_t reserved;
+  uint32_t* d_pos;          /* capacity n */
+  uint64_t* d_count;        /* device */
+} dtl_bbf_task;
+
+/* A batch plan owns the device-side task table and scratch for a fixed set
+ * of tasks; running it enqueues four kernels (clear, persistent probe over
+ * all tasks, per-task scan, expand) with no allocation or memset, so a run
+ * can be captured into a hipGraph and replayed.  Task buffers are referenced,
+ * not copied: their contents may change between runs. */
+typedef struct dtl_batch_plan dtl_batch_plan;
+
+int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
+                          const dtl_probe_params* params, dtl_batch_plan** plan);
+int bbf_batch_plan_run(dtl_batch_plan* plan, dtl_stream_t stream);
+void bbf_batch_plan_destroy(dtl_batch_plan* plan);
+
+/* Create + run + synchronize + destroy.  Each task's list equals what
+ * bbf_probe would produce for it. */
+int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_probe_params* params,
+                      dtl_stream_t stream);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char* dtl_status_string(int status);
+/* 1 if a kernel for this configuration is compiled into the library. */
+int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
+/* Number of compiled blocked Bloom filter configurations. */
+int dtl_bbf_config_count(void);
+int dtl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DTL_HIP_H */

@@ -1,0 +1,148 @@
+"""CPU-side checks of libdtl_hip: the library loads, exports every function
+include/dtl_hip.h declares, and its host-side logic (descriptor sizing and
+validation, the cuckoo host build) agrees with the oracle.  No kernel is
+launched here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import dtl_hip as D
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, re.M):
+            name = m.group(1)
+            if name not in ("if", "sizeof", "return"):
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = D.load()
+    names = declared_functions()
+    assert {"bbf_build", "bbf_probe", "cf_probe", "bbf_probe_batched"} <= names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(D.EXPORTS) <= names
+
+
+def test_version_and_status_strings():
+    L = D.load()
+    assert L.dtl_version() >= 100
+    assert D.status_string(D.EINVAL) == "invalid argument"
+
+
+def valid_configs():
+    out = []
+    for wb in (32, 64):
+        for w in (1, 2, 4, 8, 16):
+            for s in (1, 2, 4, 8, 16):
+                if s > w * wb // 8:
+                    continue
+                for k in range(1, 17):
+                    if k % s == 0:
+                        out.append((wb, w, s, 1, k))
+        for w in (4, 8, 16):
+            for z in (2, 4, 8):
+                if z >= w:
+                    continue
+                for k in range(2, 17, 2):
+                    if k % z == 0:
+                        out.append((wb, w, w, z, k))
+    return out
+
+
+def test_every_reference_configuration_is_compiled():
+    """Every configuration the reference's runtime dispatch accepts
+    (blocked_bloomfilter.hpp:169-229, zoned_blocked_bloomfilter.hpp:173-215)
+    has a kernel."""
+    L = D.load()
+    cfgs = valid_configs()
+    assert L.dtl_bbf_config_count() == len(cfgs)
+    for wb, w, s, z, k in cfgs:
+        assert L.dtl_bbf_supported(wb, w, s, z, k), (wb, w, s, z, k)
+    assert not L.dtl_bbf_supported(32, 1, 8, 1, 8)
+    assert not L.dtl_bbf_supported(32, 4, 4, 4, 8)
+
+
+@pytest.mark.parametrize("cfg", [
+    (32, 1024, 8, 1, 1, 1), (32, 1 << 20, 4, 1, 1, 1), (64, 1 << 17, 8, 8, 8, 1),
+    (64, 1 << 33, 8, 8, 8, 1), (32, 10 * 1000, 6, 16, 16, 2), (32, 10 * (1 << 22), 6, 16, 16, 2),
+    (64, 123457, 7, 4, 1, 1), (32, 3 * 512, 16, 16, 16, 8), (64, 99999, 16, 2, 16, 1),
+])
+def test_bbf_descriptor_matches_oracle(cfg):
+    wb, m, k, w, s, z = cfg
+    d = D.bbf_desc(wb, m, k, w, s, z)
+    o = O.Bloom(wb, m, k, w, s, z)
+    assert d.block_cnt == o.block_cnt
+    assert d.addr_mode == o.addr_mode
+    assert d.m_actual == o.m_actual
+    assert d.word_cnt == o.word_cnt
+    assert d.size_bytes == o.m_actual // 8
+    if d.addr_mode == D.ADDR_MAGIC:
+        assert d.magic == o.d.addr.div.magic and d.magic_shift == o.d.addr.div.shift
+    else:
+        assert d.block_shift == 32 - o.d.addr.block_cnt_log2
+
+
+def test_bbf_validation_matches_oracle():
+    rng = np.random.default_rng(0)
+    for _ in range(400):
+        wb = int(rng.choice([32, 64]))
+        w = int(rng.choice([1, 2, 3, 4, 8, 16, 32]))
+        s = int(rng.choice([1, 2, 4, 8, 16, 3]))
+        z = int(rng.choice([1, 1, 2, 4, 8, 3]))
+        k = int(rng.integers(0, 18))
+        m = int(rng.choice([1024, 1 << 20, 12345 * 8, 64, 10]))
+        assert D.bbf_desc_rc(wb, m, k, w, s, z) == O.bbf_init_rc(wb, m, k, w, s, z), (wb, m, k, w, s, z)
+
+
+@pytest.mark.parametrize("m,t,b", [(1 << 20, 16, 4), (1 << 32, 16, 4), (1000 * 64, 16, 4),
+                                   (1 << 16, 8, 2), (77777, 12, 4), (1 << 18, 32, 1)])
+def test_cf_descriptor_matches_oracle(m, t, b):
+    d = D.cf_desc(m, t, b)
+    o = O.Cuckoo(m, t, b)
+    assert d.bucket_cnt == o.bucket_cnt
+    assert d.buffer_bytes == o.buffer_bytes
+    assert d.victim_offset == o.d.victim_offset
+    assert d.m_actual == o.d.m_actual
+
+
+@pytest.mark.parametrize("t,b,load", [(16, 4, 0.95), (8, 4, 0.9), (12, 4, 0.9), (4, 2, 0.5),
+                                      (32, 2, 0.8), (16, 1, 0.4), (12, 2, 0.8)])
+def test_cuckoo_host_build_matches_oracle(t, b, load):
+    """The product's host insert (dtl_cuckoo_host.cpp) writes byte-identical
+    tables to the oracle's restatement for the same kick-RNG seed, including
+    the victim cache after overflow."""
+    m = (1 << 14) * ((t * b + 7) // 8) * 8
+    d = D.cf_desc(m, t, b)
+    o = O.Cuckoo(m, t, b)
+    rng = np.random.default_rng(42)
+    n = int(load * d.bucket_cnt * b)
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    buf, ins = D.cf_build_host(d, keys, seed=1234)
+    obuf = o.new_buffer()
+    g = O.MT19937(1234)
+    oins = sum(1 for k in keys.tolist() if O.lib().orc_cf_insert(O.C.byref(o.d), obuf.ctypes.data, k,
+                                                                  O.C.byref(g.g)))
+    assert ins == oins
+    assert np.array_equal(buf[: d.buffer_bytes], obuf[: d.buffer_bytes])
+
+
+def test_cuckoo_host_build_overflow_sets_victim():
+    d = D.cf_desc(64 * 64, 16, 4)
+    keys = (np.arange(1, 5000, dtype=np.uint64) * 2654435761 % 2**32).astype(np.uint32)
+    buf, ins = D.cf_build_host(d, keys, seed=7)
+    assert ins < len(keys)
+    assert buf[d.victim_offset + 8] == 1

@@ -1,0 +1,314 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the GPU.
+
+Bar: bit-exact.  Filter words are byte-compared with the oracle's build of the
+same keys; match lists must equal the oracle's batch_contains element for
+element, including match_offset.  Full-size filters (1 GiB) are checked
+through size-independent properties plus exact agreement on a sample.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import dtl_hip as D  # noqa: E402
+
+
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    D.load()
+    return "cuda"
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(dev())
+
+
+def mixed_keys(build, n, hit_rate, seed):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(2**31, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    nh = int(round(hit_rate * n))
+    if nh and len(build):
+        idx = rng.choice(n, nh, replace=False)
+        k[idx] = build[rng.integers(0, len(build), nh)]
+    return k
+
+
+def gpu_build(desc, keys_np):
+    words = D.new_filter(desc)
+    if len(keys_np):
+        D.bbf_build(desc, words, to_dev(keys_np))
+    torch.cuda.synchronize()
+    return words
+
+
+def oracle_words(f, build):
+    w = f.new_words()
+    f.insert(w, build)
+    return w
+
+
+def check_cfg(cfg, n_build, n_probe, seed=1, hit=0.2, offset=0, params=None):
+    wb, m, k, w, s, z = cfg
+    f = O.Bloom(wb, m, k, w, s, z)
+    desc = D.bbf_desc(wb, m, k, w, s, z)
+    rng = np.random.default_rng(seed)
+    build = rng.integers(0, 2**31, n_build, dtype=np.uint64).astype(np.uint32)
+    words = gpu_build(desc, build)
+    ow = oracle_words(f, build)
+    got = words.cpu().numpy().view(ow.dtype)
+    assert np.array_equal(got, ow), "filter words differ for %s" % (cfg,)
+    probe = mixed_keys(build, n_probe, hit, seed + 1)
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), offset, params=params)
+    torch.cuda.synchronize()
+    pos = D.positions(out, cnt)
+    exp = f.batch_contains(ow, probe, offset)
+    assert np.array_equal(pos, exp), "match list differs for %s" % (cfg,)
+    return pos
+
+
+def test_reference_kat_on_gpu():
+    """src/dtl/filter/filter_test.cpp:16-40 through bbf_build / bbf_probe."""
+    desc = D.bbf_desc(32, 1024, 8, 1, 1)
+    words = gpu_build(desc, np.array([13, 37, 42, 88], dtype=np.uint32))
+    out, cnt = D.bbf_probe(desc, words, to_dev(np.array([13, 37, 42, 88], np.uint32)), 0)
+    assert D.positions(out, cnt).tolist() == [0, 1, 2, 3]
+    out, cnt = D.bbf_probe(desc, words, to_dev(np.array([0, 1, 42, 2], np.uint32)), 1000)
+    assert D.positions(out, cnt).tolist() == [1002]
+
+
+BASE_CFGS = [
+    (32, 1 << 20, 4, 1, 1, 1),        # plain register-blocked (128 KiB, LDS)
+    (64, 1 << 17, 8, 8, 8, 1),        # cache-sectorized 16 KiB (LDS)
+    (64, 1 << 25, 8, 8, 8, 1),        # cache-sectorized 4 MiB (global/L2)
+    (32, 10 * 20000, 6, 16, 16, 2),   # zoned MAGIC (batched semi-join config)
+    (32, 10 * 700000, 6, 16, 16, 2),  # zoned MAGIC, larger than LDS
+]
+
+
+@pytest.mark.parametrize("cfg", BASE_CFGS)
+def test_benchmark_configs_small(cfg):
+    m_bits = cfg[1]
+    n_build = max(100, m_bits // 12)
+    check_cfg(cfg, min(n_build, 400000), 300000, seed=3, hit=0.1, offset=17)
+
+
+@pytest.mark.parametrize("cfg", BASE_CFGS[:2])
+def test_lds_and_global_paths_agree(cfg):
+    a = check_cfg(cfg, 5000, 100000, seed=5, params=D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL))
+    b = check_cfg(cfg, 5000, 100000, seed=5)
+    c = check_cfg(cfg, 5000, 100000, seed=5, params=D.ProbeParams(0, 1024, 7, 0, 0))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_every_configuration_parity():
+    """Every configuration the reference dispatches, POW2 and MAGIC sizes,
+    including the k=8 word-block configurations (scalar semantics)."""
+    cfgs = []
+    for wb in (32, 64):
+        for w in (1, 2, 4, 8, 16):
+            for s in (1, 2, 4, 8, 16):
+                if s > w * wb // 8:
+                    continue
+                for k in range(1, 17):
+                    if k % s == 0:
+                        cfgs.append((wb, w, s, 1, k))
+        for w in (4, 8, 16):
+            for z in (2, 4, 8):
+                if z < w:
+                    for k in range(2, 17, 2):
+                        if k % z == 0:
+                            cfgs.append((wb, w, w, z, k))
+    assert len(cfgs) == D.load().dtl_bbf_config_count()
+    for i, (wb, w, s, z, k) in enumerate(cfgs):
+        m = (1 << 16) if i % 2 == 0 else 50021
+        check_cfg((wb, m, k, w, s, z), 800, 6000, seed=100 + i, hit=0.3, offset=i)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 2047, 2048, 2049, 4097, 100003])
+def test_edge_sizes(n):
+    cfg = (64, 1 << 17, 8, 8, 8, 1)
+    desc = D.bbf_desc(*cfg[:1], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5])
+    f = O.Bloom(*cfg)
+    build = np.arange(1, 3001, dtype=np.uint32) * np.uint32(2654435761 & 0x7FFFFFFF)
+    words = gpu_build(desc, build)
+    probe = mixed_keys(build, n, 0.5, 9) if n else np.zeros(0, np.uint32)
+    keys = to_dev(probe) if n else torch.zeros(0, dtype=torch.int32, device=dev())
+    for params in (None, D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL)):
+        out, cnt = D.bbf_probe(desc, words, keys, 5, params=params)
+        torch.cuda.synchronize()
+        exp = f.batch_contains(oracle_words(f, build), probe, 5) if n else np.zeros(0, np.uint32)
+        assert np.array_equal(D.positions(out, cnt), exp)
+
+
+def test_match_offset_wraps_like_u32():
+    cfg = (32, 1 << 14, 4, 1, 1, 1)
+    off = 2**32 - 10
+    pos = check_cfg(cfg, 300, 5000, seed=11, hit=0.5, offset=off)
+    assert pos.dtype == np.uint32 and len(pos) > 20
+
+
+def test_build_is_order_independent_and_idempotent():
+    cfg = (64, 1 << 22, 8, 8, 8, 1)
+    desc = D.bbf_desc(64, 1 << 22, 8, 8, 8, 1)
+    rng = np.random.default_rng(21)
+    keys = rng.integers(0, 2**32, 300000, dtype=np.uint64).astype(np.uint32)
+    a = gpu_build(desc, keys)
+    b = gpu_build(desc, keys[::-1].copy())
+    D.bbf_build(desc, b, to_dev(keys))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    f = O.Bloom(*cfg)
+    assert np.array_equal(a.cpu().numpy().view(np.uint64), oracle_words(f, keys))
+
+
+# ---------------------------------------------------------------------------
+# cuckoo filter
+# ---------------------------------------------------------------------------
+def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False):
+    d = D.cf_desc(m, t, b)
+    o = O.Cuckoo(m, t, b)
+    rng = np.random.default_rng(seed)
+    n = int(load * d.bucket_cnt * b)
+    keys = np.unique(rng.integers(0, 2**31, int(n * 1.2) + 10, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(keys)
+    keys = keys[:n]
+    buf, ins = D.cf_build_host(d, keys, seed=seed)
+    if not overfill:
+        assert ins == len(keys)
+    dbuf = torch.from_numpy(buf).to(dev())
+    probe = mixed_keys(keys, n_probe, hit, seed + 7)
+    out, cnt = D.cf_probe(d, dbuf, to_dev(probe), offset)
+    torch.cuda.synchronize()
+    pos = D.positions(out, cnt)
+    exp = o.batch_contains(buf, probe, offset)
+    assert np.array_equal(pos, exp)
+    return d, buf, pos
+
+
+@pytest.mark.parametrize("t,b", [(16, 4), (8, 4), (12, 4), (4, 4), (32, 4), (16, 2), (8, 2), (12, 2),
+                                 (32, 2), (4, 2), (16, 1), (8, 1), (12, 1), (4, 1), (32, 1)])
+def test_cuckoo_all_configs(t, b):
+    load = {1: 0.45, 2: 0.8, 4: 0.9}[b] if t > 4 else 0.3
+    bpb = (t * b + 7) // 8
+    check_cf((1 << 13) * bpb * 8, t, b, load, 50000, 0.5, seed=t * 10 + b, overfill=(t == 4))
+    check_cf(5003 * bpb * 8, t, b, load, 30000, 0.3, seed=t * 10 + b + 1, offset=99, overfill=(t == 4))
+
+
+def test_cuckoo_victim_cache_hits():
+    d, buf, pos = check_cf(64 * 64, 16, 4, 1.5, 20000, 0.5, seed=3, overfill=True)
+    assert buf[d.victim_offset + 8] == 1
+
+
+def test_cuckoo_benchmark_config_small():
+    """Config 3 shape at 1/64 scale: 16-bit tags, 4 slots, pow2, 95% load."""
+    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.01, seed=44)
+    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.5, seed=45)
+
+
+# ---------------------------------------------------------------------------
+# batched
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("params", [None, D.ProbeParams(0, 0, 0, 4, 0), D.ProbeParams(0, 1024, 0, 2, 0),
+                                    D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL),
+                                    D.ProbeParams(100000, 1024, 64, 8, 0)])
+def test_batched_matches_per_filter(params):
+    rng = np.random.default_rng(77)
+    tasks, expect = [], []
+    for i in range(40):
+        n = int(2 ** rng.uniform(6, 16))
+        m = 10 * n
+        f = O.Bloom(32, m, 6, 16, 16, 2)
+        desc = D.bbf_desc(32, m, 6, 16, 16, 2)
+        build = rng.integers(0, 2**31, n, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        nprobe = int(2 ** rng.uniform(0, 15)) if i % 9 else 0
+        probe = mixed_keys(build, nprobe, rng.uniform(0.001, 0.3), i)
+        keys = to_dev(probe) if nprobe else torch.zeros(0, dtype=torch.int32, device=dev())
+        off = int(rng.integers(0, 1 << 20))
+        tasks.append((desc, words, keys, off))
+        expect.append(f.batch_contains(oracle_words(f, build), probe, off) if nprobe else np.zeros(0, np.uint32))
+    plan = D.BatchPlan(tasks, params)
+    for rep in range(2):
+        plan.run()
+        torch.cuda.synchronize()
+        got = plan.results()
+        for g, e in zip(got, expect):
+            assert np.array_equal(g, e)
+    plan.close()
+
+
+def test_batched_graph_replay():
+    """The batch plan runs under hipGraph capture; replays match the eager run."""
+    rng = np.random.default_rng(5)
+    tasks, expect = [], []
+    for i in range(12):
+        n = int(2 ** rng.uniform(8, 14))
+        f = O.Bloom(32, 10 * n, 6, 16, 16, 2)
+        desc = D.bbf_desc(32, 10 * n, 6, 16, 16, 2)
+        build = rng.integers(0, 2**31, n, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        probe = mixed_keys(build, 5000 + 100 * i, 0.1, i)
+        tasks.append((desc, words, to_dev(probe), i))
+        expect.append(f.batch_contains(oracle_words(f, build), probe, i))
+    plan = D.BatchPlan(tasks)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.run(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.run(s)
+    for rep in range(3):
+        for c in plan.counts:
+            c.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for got, e in zip(plan.results(), expect):
+            assert np.array_equal(got, e)
+    plan.close()
+
+
+# ---------------------------------------------------------------------------
+# full size (1 GiB cache-sectorized filter): properties + exact sample
+# ---------------------------------------------------------------------------
+def test_full_size_1gib_properties():
+    cfg = (64, 1 << 33, 8, 8, 8, 1)
+    desc = D.bbf_desc(*[cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]])
+    f = O.Bloom(*cfg)
+    n_build = (1 << 33) // 12
+    g = torch.Generator(device="cuda").manual_seed(1)
+    build = torch.randint(0, 2**31, (n_build,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+    words = D.new_filter(desc)
+    D.bbf_build(desc, words, build)
+    n_probe = 1 << 28
+    probe = torch.randint(-(2**31), 0, (n_probe,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+    hits = int(0.01 * n_probe)
+    idx = torch.randperm(n_probe, device="cuda", generator=g)[:hits]
+    probe[idx] = build[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
+    out, cnt = D.bbf_probe(desc, words, probe, 0)
+    torch.cuda.synchronize()
+    c = int(cnt.item())
+    pos = out[:c]
+    assert c >= hits
+    assert bool((pos[1:] > pos[:-1]).all())  # strictly ascending
+    # every planted hit is reported (no false negatives)
+    isin = torch.zeros(n_probe, dtype=torch.bool, device="cuda")
+    isin[pos.long()] = True
+    assert bool(isin[idx].all())
+    fpr = (c - hits) / (n_probe - hits)
+    model = O.fpr_config(desc.m_actual, n_build, 8, 8, 8, 8, 1)
+    assert abs(fpr - model) / model < 0.05, (fpr, model)
+    # exact agreement with the oracle on a sample of keys
+    host_words = words.cpu().numpy().view(np.uint64)
+    sample = torch.randint(0, n_probe, (200000,), device="cuda", generator=g)
+    sk = probe[sample].cpu().numpy().view(np.uint32)
+    si = isin[sample].cpu().numpy()
+    for key, flag in zip(sk[:50000].tolist(), si[:50000].tolist()):
+        assert f.contains(host_words, key) == flag
+    bs = build[torch.randint(0, n_build, (20000,), device="cuda", generator=g)].cpu().numpy().view(np.uint32)
+    assert all(f.contains(host_words, k) for k in bs.tolist())
