@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--hit-rate", type=float, default=0.01)
     ap.add_argument("--cpu-sample", type=int, default=1 << 26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box in this setup has a single MI355X, "
@@ -127,6 +129,12 @@ def main():
     D.load()
     torch.cuda.set_device(0)
 
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("bench: building inputs (m=2^%d bits, 2^%d probe keys)" % (args.log2_m, args.log2_probe))
     desc = D.bbf_desc(64, 1 << args.log2_m, 8, 8, 8, 1)
     build, probe, n_build = make_inputs(torch, args.log2_m, 1 << args.log2_probe, args.hit_rate, 1)

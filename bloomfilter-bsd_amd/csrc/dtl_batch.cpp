@@ -134,11 +134,15 @@ int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   BbfBatchLaunch L = p->launch;
   L.stream = st;
-  if (zero_u32(L.work_counter, st) != hipSuccess) return DTL_EHIP;
-  if (p->total_tiles > 0 && p->entry->probe_batched(L) != hipSuccess) return DTL_EHIP;
-  if (compact_tasks(L.bits, L.tile_cnt, L.tasks, L.tile_task, p->n_tasks, p->total_tiles, st) !=
-      hipSuccess)
+  (void)hipGetLastError();  // report only this call's failures
+  hipError_t e = zero_u32(L.work_counter, st);
+  if (e == hipSuccess && p->total_tiles > 0) e = p->entry->probe_batched(L);
+  if (e == hipSuccess)
+    e = compact_tasks(L.bits, L.tile_cnt, L.tasks, L.tile_task, p->n_tasks, p->total_tiles, st);
+  if (e != hipSuccess) {
+    set_last_error(e);
     return DTL_EHIP;
+  }
   return DTL_OK;
 }
 

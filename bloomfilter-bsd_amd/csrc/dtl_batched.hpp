@@ -113,10 +113,9 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
 template <class C, int BT>
 hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_counter, uint32_t chunk) {
   auto fn = k_bbf_probe_batched<C, BT>;
-  static const hipError_t attr = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  (void)attr;
   const uint32_t lds_bytes = (L.lds_max_bytes + 15u) & ~15u;
+  hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
+  if (ae != hipSuccess) return ae;
   uint32_t grid = L.grid;
   if (grid == 0) {
     int per_cu = 0;

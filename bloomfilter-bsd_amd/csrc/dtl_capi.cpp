@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -185,7 +186,21 @@ static int get_scratch(int dev, hipStream_t stream, size_t bytes, void** out) {
   return DTL_OK;
 }
 
-static int hip_status(hipError_t e) { return e == hipSuccess ? DTL_OK : DTL_EHIP; }
+static thread_local char g_last_error[256] = "";
+
+void set_last_error(hipError_t e) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s (%d)", hipGetErrorString(e), static_cast<int>(e));
+}
+
+static int hip_status(hipError_t e) {
+  if (e == hipSuccess) return DTL_OK;
+  set_last_error(e);
+  return DTL_EHIP;
+}
+
+// Errors of earlier, unrelated HIP calls stay in the runtime's last-error
+// slot; clear it so that a launch check reports only this call's failures.
+static void clear_stale_error() { (void)hipGetLastError(); }
 
 static void fill_addr(const dtl_bbf_desc* d, uint32_t* magic, uint32_t* mshift, uint32_t* divisor,
                       uint32_t* post) {
@@ -220,6 +235,8 @@ using namespace dtlh;
 extern "C" {
 
 int dtl_version(void) { return 100; }
+
+const char* dtl_last_error(void) { return g_last_error; }
 
 const char* dtl_status_string(int s) {
   switch (s) {
@@ -325,6 +342,7 @@ int bbf_build(const dtl_bbf_desc* d, void* d_words, const uint32_t* d_keys, uint
   const BbfEntry* e = bbf_find(d->word_bits, d->w, d->s, d->z, d->k);
   if (!e) return DTL_EINVAL;
   if (n == 0) return DTL_OK;
+  clear_stale_error();
   DeviceInfo di;
   int rc = current_device_info(&di, nullptr);
   if (rc != DTL_OK) return rc;
@@ -347,6 +365,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   const BbfEntry* e = bbf_find(d->word_bits, d->w, d->s, d->z, d->k);
   if (!e) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  clear_stale_error();
   DeviceInfo di;
   int dev = 0;
   int rc = current_device_info(&di, &dev);
@@ -381,7 +400,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
       hipEventRecord(static_cast<hipEvent_t>(params->ev_start), st) != hipSuccess)
     return DTL_EHIP;
   hipError_t he = e->probe(L);
-  if (he != hipSuccess) return DTL_EHIP;
+  if (he != hipSuccess) return hip_status(he);
   if (params && params->ev_stop &&
       hipEventRecord(static_cast<hipEvent_t>(params->ev_stop), st) != hipSuccess)
     return DTL_EHIP;
@@ -401,6 +420,7 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   const CfEntry* e = cf_find(d->tag_bits, d->tags_per_bucket);
   if (!e) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  clear_stale_error();
   DeviceInfo di;
   int dev = 0;
   int rc = current_device_info(&di, &dev);
@@ -427,7 +447,8 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   L.tile_cnt = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + bits_bytes);
   L.ntiles = ntiles;
   L.stream = st;
-  if (e->probe(L) != hipSuccess) return DTL_EHIP;
+  hipError_t he = e->probe(L);
+  if (he != hipSuccess) return hip_status(he);
   return hip_status(compact_single(L.bits, L.tile_cnt, ntiles, L.n, match_offset, d_pos, d_count, st));
 }
 

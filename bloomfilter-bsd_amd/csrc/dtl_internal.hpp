@@ -129,6 +129,16 @@ hipError_t compact_tasks(const uint64_t* bits, uint32_t* tile_cnt, const BatchTa
 hipError_t write_zero_count(uint64_t* d_count, hipStream_t stream);
 hipError_t zero_u32(uint32_t* p, hipStream_t stream);
 
+// Raises a kernel's dynamic-LDS limit when `bytes` exceeds the default
+// 64 KiB (gfx950 workgroups may use up to 160 KiB).
+inline hipError_t allow_dynamic_lds(const void* fn, uint32_t bytes) {
+  if (bytes <= 64u * 1024u) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+}
+
+// Records the text of a failed HIP call for dtl_last_error().
+void set_last_error(hipError_t e);
+
 // Device info of the current device (CU count, LDS bytes per workgroup).
 int current_device(int* dev, int* cu_count, uint32_t* max_lds);
 

@@ -26,10 +26,9 @@ template <class C, int BT>
 hipError_t launch_probe_lds(const BbfProbeLaunch& L) {
   using W = typename C::word_t;
   auto fn = k_bbf_probe_lds<C, BT>;
-  static const hipError_t attr = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  (void)attr;
   const uint32_t lds_bytes = (L.filter_bytes + 15u) & ~15u;
+  hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
+  if (ae != hipSuccess) return ae;
   uint32_t grid = L.lds_grid;
   if (grid == 0) {
     int per_cu = 0;

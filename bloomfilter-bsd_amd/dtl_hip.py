@@ -48,15 +48,16 @@ class BbfTask(C.Structure):
 
 class DtlError(RuntimeError):
     def __init__(self, fn, status):
-        super().__init__("%s failed: %s (%d)" % (fn, status_string(status), status))
+        detail = load().dtl_last_error().decode() if status == EHIP else ""
+        super().__init__("%s failed: %s (%d) %s" % (fn, status_string(status), status, detail))
         self.status = status
 
 
 EXPORTS = [
     "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
-    "bbf_probe_batched", "dtl_status_string", "dtl_bbf_supported", "dtl_bbf_config_count",
-    "dtl_version",
+    "bbf_probe_batched", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
+    "dtl_bbf_config_count", "dtl_version",
 ]
 
 _lib = None
@@ -87,6 +88,8 @@ def load(path=LIB_PATH):
     L.bbf_probe_batched.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), vp]
     L.dtl_status_string.argtypes = [i32]
     L.dtl_status_string.restype = C.c_char_p
+    L.dtl_last_error.argtypes = []
+    L.dtl_last_error.restype = C.c_char_p
     L.dtl_bbf_supported.argtypes = [u32, u32, u32, u32, u32]
     L.dtl_bbf_config_count.argtypes = []
     L.dtl_version.argtypes = []

@@ -49,6 +49,7 @@ inline void throw_status(int rc, const std::string& what) {
     case DTL_OK: return;
     case DTL_EINVAL: throw std::invalid_argument(what);
     case DTL_ELOGIC: throw std::logic_error("The block count must not exceed 2^32.");
+    case DTL_EHIP: throw std::runtime_error(what + ": " + dtl_status_string(rc) + ": " + dtl_last_error());
     default: throw std::runtime_error(what + ": " + dtl_status_string(rc));
   }
 }

@@ -176,6 +176,8 @@ int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_pro
 
 /* ---- misc ---------------------------------------------------------------- */
 const char* dtl_status_string(int status);
+/* Text of the last failed HIP call of this thread inside the library. */
+const char* dtl_last_error(void);
 /* 1 if a kernel for this configuration is compiled into the library. */
 int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
 /* Number of compiled blocked Bloom filter configurations. */
