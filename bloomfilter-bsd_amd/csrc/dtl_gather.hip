@@ -1,0 +1,93 @@
+// dtl_gather.hip -- random-sector read microbenchmark (roofline reference).
+//
+// Reads n uniformly scattered, aligned sectors of a device buffer the way a
+// filter probe reads its blocks, and reports nothing but time: it measures
+// the achievable random-access bandwidth the probe kernels are compared
+// with.  Access patterns (mode):
+//   0  one lane per 64 B sector, four 16-byte loads
+//   1  as 0 with non-temporal loads
+//   2  four lanes per 64 B sector, one 16-byte load each (one 64 B request)
+//   3  as 2 with non-temporal loads
+//   4  one lane per 32 B sector, two 16-byte loads
+//   5  two lanes per 32 B sector, one 16-byte load each
+#include <hip/hip_runtime.h>
+
+#include "dtl_hip.h"
+#include "dtl_internal.hpp"
+
+namespace dtlh {
+
+constexpr int kGatherPerLane = 8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t sector_of(uint64_t i, uint32_t shift) {
+  return (static_cast<uint32_t>(i) * 0x9E3779B1u) >> shift;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, uint32_t shift,
+                                                uint64_t n_reads, uint64_t* __restrict__ sink) {
+  constexpr bool NT = (MODE == 1 || MODE == 3);
+  constexpr int LANES = (MODE == 2 || MODE == 3) ? 4 : (MODE == 5 ? 2 : 1);
+  constexpr int V = (MODE == 4) ? 2 : ((MODE == 5 || LANES == 4) ? 1 : 4);  // uint4 per lane per sector
+  constexpr int SEC_V = (MODE == 4 || MODE == 5) ? 2 : 4;                    // uint4 per sector
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t group = gid / LANES;
+  const uint32_t part = static_cast<uint32_t>(gid % LANES);
+  u32x4 acc = {0u, 0u, 0u, 0u};
+  u32x4 v[kGatherPerLane][V];
+#pragma unroll
+  for (int r = 0; r < kGatherPerLane; ++r) {
+    const uint64_t i = group * kGatherPerLane + r;
+    const uint64_t s = i < n_reads ? sector_of(i, shift) : 0;
+    const u32x4* p = buf + s * SEC_V + part * V;
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[r][j] = NT ? __builtin_nontemporal_load(p + j) : p[j];
+  }
+#pragma unroll
+  for (int r = 0; r < kGatherPerLane; ++r)
+#pragma unroll
+    for (int j = 0; j < V; ++j) acc ^= v[r][j];
+  if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9E3779B1u) sink[0] = gid;  // keeps the loads alive
+}
+
+template <int MODE>
+static hipError_t launch_gather(const void* buf, uint32_t shift, uint64_t n, uint64_t* sink,
+                                hipStream_t st) {
+  constexpr int LANES = (MODE == 2 || MODE == 3) ? 4 : (MODE == 5 ? 2 : 1);
+  const uint64_t threads = (n + kGatherPerLane - 1) / kGatherPerLane * LANES;
+  const uint64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL(k_gather<MODE>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st,
+                     static_cast<const u32x4*>(buf), shift, n, sink);
+  return hipGetLastError();
+}
+
+}  // namespace dtlh
+
+extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
+                                uint64_t* d_sink, dtl_stream_t stream) {
+  using namespace dtlh;
+  const uint64_t sector = (mode == 4 || mode == 5) ? 32 : 64;
+  if (!d_buf || !d_sink || mode > 5 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
+    return DTL_EINVAL;
+  if ((n_reads + kGatherPerLane - 1) / kGatherPerLane * 4 / 256 > 0x7FFFFFFFull) return DTL_EINVAL;
+  const uint64_t sectors = buf_bytes / sector;
+  const uint32_t shift = 32u - (63u - static_cast<uint32_t>(__builtin_clzll(sectors)));
+  if (sectors > (1ull << 32)) return DTL_EINVAL;
+  (void)hipGetLastError();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  switch (mode) {
+    case 0: e = launch_gather<0>(d_buf, shift, n_reads, d_sink, st); break;
+    case 1: e = launch_gather<1>(d_buf, shift, n_reads, d_sink, st); break;
+    case 2: e = launch_gather<2>(d_buf, shift, n_reads, d_sink, st); break;
+    case 3: e = launch_gather<3>(d_buf, shift, n_reads, d_sink, st); break;
+    case 4: e = launch_gather<4>(d_buf, shift, n_reads, d_sink, st); break;
+    default: e = launch_gather<5>(d_buf, shift, n_reads, d_sink, st); break;
+  }
+  if (e != hipSuccess) {
+    set_last_error(e);
+    return DTL_EHIP;
+  }
+  return DTL_OK;
+}

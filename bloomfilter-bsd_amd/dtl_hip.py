@@ -57,7 +57,7 @@ EXPORTS = [
     "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
-    "dtl_bbf_config_count", "dtl_version",
+    "dtl_bbf_config_count", "dtl_version", "dtl_gather_bench",
 ]
 
 _lib = None
@@ -93,9 +93,11 @@ def load(path=LIB_PATH):
     L.dtl_bbf_supported.argtypes = [u32, u32, u32, u32, u32]
     L.dtl_bbf_config_count.argtypes = []
     L.dtl_version.argtypes = []
+    L.dtl_gather_bench.argtypes = [vp, u64, u32, u64, vp, vp]
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
                  "cf_probe", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
-                 "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_version"):
+                 "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_version",
+                 "dtl_gather_bench"):
         getattr(L, name).restype = i32
     _lib = L
     return L
@@ -241,3 +243,8 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def gather_bench(buf, mode, n_reads, sink, stream=None):
+    _check("dtl_gather_bench", load().dtl_gather_bench(_ptr(buf), buf.numel() * buf.element_size(), mode,
+                                                       n_reads, _ptr(sink), _stream(stream)))

@@ -174,6 +174,15 @@ void bbf_batch_plan_destroy(dtl_batch_plan* plan);
 int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_probe_params* params,
                       dtl_stream_t stream);
 
+/* ---- measurement ----------------------------------------------------------- */
+/* Random-sector read microbenchmark (no reference counterpart; used to
+ * measure the random-access bandwidth the probe kernels are compared with).
+ * Reads n_reads pseudo-random aligned sectors (64 B, or 32 B for modes 4-5)
+ * of a buffer of buf_bytes (a power of two); the access pattern `mode` is
+ * described in dtl_gather.hip.  d_sink is a device uint64_t scratch word. */
+int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
+                     uint64_t* d_sink, dtl_stream_t stream);
+
 /* ---- misc ---------------------------------------------------------------- */
 const char* dtl_status_string(int status);
 /* Text of the last failed HIP call of this thread inside the library. */

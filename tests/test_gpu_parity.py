@@ -179,6 +179,7 @@ def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False):
     buf, ins = D.cf_build_host(d, keys, seed=seed)
     if not overfill:
         assert ins == len(keys)
+    assert ins > 0
     dbuf = torch.from_numpy(buf).to(dev())
     probe = mixed_keys(keys, n_probe, hit, seed + 7)
     out, cnt = D.cf_probe(d, dbuf, to_dev(probe), offset)
@@ -192,10 +193,13 @@ def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False):
 @pytest.mark.parametrize("t,b", [(16, 4), (8, 4), (12, 4), (4, 4), (32, 4), (16, 2), (8, 2), (12, 2),
                                  (32, 2), (4, 2), (16, 1), (8, 1), (12, 1), (4, 1), (32, 1)])
 def test_cuckoo_all_configs(t, b):
-    load = {1: 0.45, 2: 0.8, 4: 0.9}[b] if t > 4 else 0.3
+    load = {1: 0.3, 2: 0.8, 4: 0.9}[b] if t > 4 else 0.3
     bpb = (t * b + 7) // 8
-    check_cf((1 << 13) * bpb * 8, t, b, load, 50000, 0.5, seed=t * 10 + b, overfill=(t == 4))
-    check_cf(5003 * bpb * 8, t, b, load, 30000, 0.3, seed=t * 10 + b + 1, offset=99, overfill=(t == 4))
+    # tables with one slot per bucket or 4-bit tags may overflow into the
+    # victim cache; the probe must still match the oracle on the same bytes
+    over = t == 4 or b == 1
+    check_cf((1 << 13) * bpb * 8, t, b, load, 50000, 0.5, seed=t * 10 + b, overfill=over)
+    check_cf(5003 * bpb * 8, t, b, load, 30000, 0.3, seed=t * 10 + b + 1, offset=99, overfill=over)
 
 
 def test_cuckoo_victim_cache_hits():
