@@ -98,6 +98,148 @@ def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
                          len(one) / secs1 / 1e9)}
 
 
+GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal",
+                2: "4 lanes / 64 B sector, 1x16 B each", 3: "as 2, non-temporal",
+                4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector"}
+
+
+def gather_table(torch, D, modes, n=1 << 28, reps=5):
+    """Random-sector read microbenchmark over a 1 GiB buffer (dtl_gather_bench)."""
+    buf = torch.randint(0, 2**31, (1 << 28,), dtype=torch.int32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.int64, device="cuda")
+    out = {}
+    for mode in modes:
+        sector = 32 if mode >= 4 else 64
+        for _ in range(2):
+            D.gather_bench(buf, mode, n, sink)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            D.gather_bench(buf, mode, n, sink)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        out[mode] = {"pattern": GATHER_MODES[mode], "ms": round(ms, 4),
+                     "G_sectors_per_s": round(n / ms / 1e6, 2),
+                     "useful_GB_per_s": round(n * sector / ms / 1e6, 1)}
+    del buf
+    return out
+
+
+def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, warmup, label):
+    """Whole-job probe rate of one filter configuration and size."""
+    desc = D.bbf_desc(word_bits, 1 << log2_m, k, w, s, z)
+    build, probe, n_build = make_inputs(torch, log2_m, n_probe, hit_rate, 7)
+    if word_bits == 32 and w == 1:  # the plain filter is built from 2^16 keys (its own config)
+        build = build[: 1 << 16]
+    words = D.new_filter(desc)
+    D.bbf_build(desc, words, build)
+    torch.cuda.synchronize()
+    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup)
+    return {"config": label, "filter_bytes": int(desc.size_bytes),
+            "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
+            "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
+            "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
+            "matches": matches}
+
+
+def batched_workload(torch, D, n_filters, seed, cfg=(32, 6, 16, 16, 2), bits_per_key=10,
+                     log2_build=(10, 22), log2_probe=(12, 24), hit=(0.001, 0.30)):
+    """The batched semi-join workload: n_filters zoned blocked Bloom filters
+    (u32 words, w=16, z=2, k=6 -> MAGIC addressing at m = 10 bits/key), build
+    sizes log-uniform in 2^10..2^22 keys, per-filter probe batches log-uniform
+    in 2^12..2^24 keys with hit rates uniform in [0.1%, 30%]; seeded."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    wb, k, w, s_, z = cfg
+    tasks, meta = [], []
+    for f in range(n_filters):
+        nb = int(2 ** rng.uniform(*log2_build))
+        npb = int(2 ** rng.uniform(*log2_probe))
+        h = float(rng.uniform(*hit))
+        desc = D.bbf_desc(wb, bits_per_key * nb, k, w, s_, z)
+        build = torch.randint(0, 2**31, (nb,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+        probe = torch.randint(-(2**31), 0, (npb,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+        hits = int(round(h * npb))
+        if hits:
+            idx = torch.randperm(npb, device="cuda", generator=g)[:hits]
+            probe[idx] = build[torch.randint(0, nb, (hits,), device="cuda", generator=g)]
+        words = D.new_filter(desc)
+        D.bbf_build(desc, words, build)
+        tasks.append((desc, words, probe, 0))
+        meta.append({"n_build": nb, "n_probe": npb, "hits": hits})
+    torch.cuda.synchronize()
+    return tasks, meta
+
+
+def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0)):
+    """Plain per-filter launches of the fixed-config kernel vs one autotuned,
+    graph-captured batch plan over all filters (same filters, same lists)."""
+    log("batched: generating %d filters" % n_filters)
+    tasks, meta = batched_workload(torch, D, n_filters, seed)
+    n_keys = sum(m["n_probe"] for m in meta)
+    true_hits = sum(m["hits"] for m in meta)
+    outs = [torch.empty(max(t[2].numel(), 1), dtype=torch.int32, device="cuda") for t in tasks]
+    cnts = [torch.zeros(1, dtype=torch.int64, device="cuda") for _ in tasks]
+    stream = torch.cuda.current_stream()
+
+    def plain_once():
+        for (desc, words, keys, off), o, c in zip(tasks, outs, cnts):
+            D.bbf_probe(desc, words, keys, off, out=o, count=c)
+
+    plain_once()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        plain_once()
+    b.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = a.elapsed_time(b) / steps
+    plain_matches = sum(int(c.item()) for c in cnts)
+
+    log("batched: autotuning the batch plan")
+    plan = D.BatchPlan(tasks)
+    best = plan.tune(reps=3)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.run(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        plan.run(s)
+    graph.replay()
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(steps):
+        graph.replay()
+    b.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = a.elapsed_time(b) / steps
+    batch_matches = sum(int(c.item()) for c in plan.counts)
+    same = all(torch.equal(c1, c2) for c1, c2 in zip(cnts, plan.counts))
+    plan.close()
+    fp = plain_matches - true_hits
+    res = {"filters": n_filters, "probe_keys": n_keys,
+           "filter_bytes": int(sum(t[0].size_bytes for t in tasks)),
+           "plain_ms": round(plain_ms, 3), "batched_graph_ms": round(batch_ms, 3),
+           "plain_G_probes_per_s": round(n_keys / plain_ms / 1e6, 2),
+           "batched_G_probes_per_s": round(n_keys / batch_ms / 1e6, 2),
+           "speedup": round(plain_ms / batch_ms, 3),
+           "matches": plain_matches, "false_positives": fp, "counts_identical": bool(same and batch_matches == plain_matches),
+           "tuned": {"lds_max_bytes": best.lds_max_bytes, "block_threads": best.block_threads,
+                     "chunk_tiles": best.chunk_tiles},
+           "total_filter_cost_ms": {("tw=%gns" % t): {"plain": round(plain_ms + t * fp * 1e-6, 3),
+                                                      "batched": round(batch_ms + t * fp * 1e-6, 3)}
+                                    for t in tw_ns}}
+    del tasks, outs, cnts
+    torch.cuda.empty_cache()
+    return res
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -116,6 +258,11 @@ def main():
     ap.add_argument("--hit-rate", type=float, default=0.01)
     ap.add_argument("--cpu-sample", type=int, default=1 << 26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gather-only", action="store_true",
+                    help="only run the random-sector gather microbenchmark (all access patterns)")
+    ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
+    ap.add_argument("--batched", type=int, default=0, metavar="FILTERS",
+                    help="only run the batched semi-join workload with this many filters (4096 = full)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -128,6 +275,15 @@ def main():
     import dtl_hip as D
     D.load()
     torch.cuda.set_device(0)
+
+    if args.batched:
+        print(json.dumps({"batched_semi_join": run_batched(torch, D, args.batched, max(2, args.steps // 4))}),
+              flush=True)
+        return
+
+    if args.gather_only:
+        print(json.dumps({"random_sector_gather_1GiB": gather_table(torch, D, range(6))}), flush=True)
+        return
 
     if args.calib_copy:
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
@@ -189,6 +345,23 @@ def main():
         import oracle as O
         log("bench: CPU baseline on %d keys" % args.cpu_sample)
         result["cpu_baseline"] = cpu_baseline(torch, O, CB, words, probe, args.log2_m, args.cpu_sample)
+    del probe, words
+    torch.cuda.empty_cache()
+    g = gather_table(torch, D, [0, 2])
+    best = max(g.values(), key=lambda r: r["G_sectors_per_s"])
+    result["roofline"]["measured_random_64B_gather"] = {
+        "G_sectors_per_s": best["G_sectors_per_s"], "pattern": best["pattern"],
+        "probe_kernel_vs_gather": round(n / (kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
+        "note": "gfx950 fills a whole 128 B L2 line per random 64 B block (TCC_EA0_RDREQ_128B = all reads)"}
+    if not args.no_regimes:
+        log("bench: filter-size regimes")
+        steps = max(3, args.steps // 2)
+        regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 16 KiB"),
+                regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 4 MiB"),
+                regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 128 MiB"),
+                regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, 1,
+                       "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
+        result["regimes"] = regs
     print(json.dumps(result), flush=True)
 
 

@@ -148,6 +148,71 @@ int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
 
 void bbf_batch_plan_destroy(dtl_batch_plan* p) { plan_free(p); }
 
+// Autotuner: times every candidate launch shape of the persistent probe
+// kernel on the plan's own tasks (median of `reps` runs, HIP events on
+// `stream`) and keeps the fastest.  Candidates: LDS staging threshold
+// {0, 8, 16, 32, 64, 96, 150 KiB} x workgroup {256, 1024} threads x tiles per
+// grab {1, 2, 4, 8, 16}; the grid is CUs x occupancy.
+int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
+                        dtl_probe_params* best_out) {
+  if (!p || reps == 0) return DTL_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int dev = 0, cus = 0;
+  uint32_t max_lds = 0;
+  int rc = current_device(&dev, &cus, &max_lds);
+  if (rc != DTL_OK) return rc;
+  const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
+  const uint32_t lds_opts[] = {0, 8 << 10, 16 << 10, 32 << 10, 64 << 10, 96 << 10, 150 << 10};
+  const uint32_t thr_opts[] = {256, 1024};
+  const uint32_t chunk_opts[] = {1, 2, 4, 8, 16};
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return DTL_EHIP;
+  const BbfBatchLaunch saved = p->launch;
+  float best_ms = 1e30f;
+  BbfBatchLaunch best = saved;
+  rc = DTL_OK;
+  for (uint32_t lds : lds_opts) {
+    if (lds > lds_cap) continue;
+    for (uint32_t thr : thr_opts) {
+      for (uint32_t ch : chunk_opts) {
+        p->launch.lds_max_bytes = lds;
+        p->launch.threads = thr;
+        p->launch.chunk = ch;
+        p->launch.grid = 0;
+        std::vector<float> t;
+        for (uint32_t r = 0; r < reps + 1 && rc == DTL_OK; ++r) {
+          if (hipEventRecord(a, st) != hipSuccess) rc = DTL_EHIP;
+          if (rc == DTL_OK) rc = bbf_batch_plan_run(p, stream);
+          if (rc == DTL_OK && hipEventRecord(b, st) != hipSuccess) rc = DTL_EHIP;
+          if (rc == DTL_OK && hipEventSynchronize(b) != hipSuccess) rc = DTL_EHIP;
+          float ms = 0;
+          if (rc == DTL_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = DTL_EHIP;
+          if (r > 0) t.push_back(ms);  // the first run warms up
+        }
+        if (rc != DTL_OK) break;
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (med < best_ms) {
+          best_ms = med;
+          best = p->launch;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  p->launch = (rc == DTL_OK) ? best : saved;
+  if (best_out) {
+    std::memset(best_out, 0, sizeof(*best_out));
+    best_out->lds_max_bytes = p->launch.lds_max_bytes;
+    best_out->block_threads = p->launch.threads;
+    best_out->grid_blocks = p->launch.grid;
+    best_out->chunk_tiles = p->launch.chunk;
+    best_out->flags = p->launch.lds_max_bytes == 0 ? DTL_PROBE_FORCE_GLOBAL : 0;
+  }
+  return rc;
+}
+
 int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_probe_params* params,
                       dtl_stream_t stream) {
   dtl_batch_plan* p = nullptr;

@@ -56,7 +56,7 @@ class DtlError(RuntimeError):
 EXPORTS = [
     "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
-    "bbf_probe_batched", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
+    "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
     "dtl_bbf_config_count", "dtl_version", "dtl_gather_bench",
 ]
 
@@ -86,6 +86,8 @@ def load(path=LIB_PATH):
     L.bbf_batch_plan_destroy.argtypes = [vp]
     L.bbf_batch_plan_destroy.restype = None
     L.bbf_probe_batched.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), vp]
+    L.bbf_batch_plan_tune.argtypes = [vp, vp, u32, C.POINTER(ProbeParams)]
+    L.bbf_batch_plan_tune.restype = i32
     L.dtl_status_string.argtypes = [i32]
     L.dtl_status_string.restype = C.c_char_p
     L.dtl_last_error.argtypes = []
@@ -229,6 +231,11 @@ class BatchPlan:
 
     def run(self, stream=None):
         _check("bbf_batch_plan_run", self.L.bbf_batch_plan_run(self.plan, _stream(stream)))
+
+    def tune(self, reps=3, stream=None):
+        best = ProbeParams()
+        _check("bbf_batch_plan_tune", self.L.bbf_batch_plan_tune(self.plan, _stream(stream), reps, C.byref(best)))
+        return best
 
     def results(self):
         return [positions(o, c) for o, c in zip(self.outs, self.counts)]

@@ -169,6 +169,12 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
 int bbf_batch_plan_run(dtl_batch_plan* plan, dtl_stream_t stream);
 void bbf_batch_plan_destroy(dtl_batch_plan* plan);
 
+/* Autotunes the plan's launch shape (LDS staging threshold, workgroup size,
+ * tiles per grab) by timing `reps` runs of each candidate on the plan's own
+ * tasks; the fastest shape is kept for later runs and copied to *best. */
+int bbf_batch_plan_tune(dtl_batch_plan* plan, dtl_stream_t stream, uint32_t reps,
+                        dtl_probe_params* best);
+
 /* Create + run + synchronize + destroy.  Each task's list equals what
  * bbf_probe would produce for it. */
 int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_probe_params* params,

@@ -316,3 +316,25 @@ def test_full_size_1gib_properties():
         assert f.contains(host_words, key) == flag
     bs = build[torch.randint(0, n_build, (20000,), device="cuda", generator=g)].cpu().numpy().view(np.uint32)
     assert all(f.contains(host_words, k) for k in bs.tolist())
+
+
+def test_batched_autotune_keeps_results():
+    rng = np.random.default_rng(123)
+    tasks, expect = [], []
+    for i in range(20):
+        n = int(2 ** rng.uniform(8, 15))
+        f = O.Bloom(32, 10 * n, 6, 16, 16, 2)
+        desc = D.bbf_desc(32, 10 * n, 6, 16, 16, 2)
+        build = rng.integers(0, 2**31, n, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        probe = mixed_keys(build, int(2 ** rng.uniform(10, 16)), 0.05, i)
+        tasks.append((desc, words, to_dev(probe), 3 * i))
+        expect.append(f.batch_contains(oracle_words(f, build), probe, 3 * i))
+    plan = D.BatchPlan(tasks)
+    best = plan.tune(reps=1)
+    assert best.block_threads in (256, 1024) and best.chunk_tiles >= 1
+    plan.run()
+    torch.cuda.synchronize()
+    for g, e in zip(plan.results(), expect):
+        assert np.array_equal(g, e)
+    plan.close()
