@@ -29,14 +29,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def distinct_keys(torch, n, seed, high_bit):
+    """n distinct uint32 keys (as int32) from a seeded bijection of [0, 2^31):
+    x = (i + 2^20 * seed) mod 2^31, then two odd multiplications and xor-shifts
+    modulo 2^31 (each step is invertible), bit 31 set to `high_bit`.  Build
+    keys use high_bit = 0, keys known to be absent use high_bit = 1."""
+    M = (1 << 31) - 1
+    x = (torch.arange(n, dtype=torch.int64, device="cuda") + (seed << 20)) & M
+    x = (x * 0x5851F42D) & M
+    x = x ^ (x >> 16)
+    x = (x * 0x2C1B3C6D) & M
+    x = x ^ (x >> 13)
+    x = (x * 0x297A2D39) & M
+    if high_bit:
+        x = x | (1 << 31)
+    return x.to(torch.int32)
+
+
 def make_inputs(torch, log2_m, n_probe, hit_rate, seed):
-    """Seeded synthetic inputs on the GPU: build keys uniform in [0, 2^31),
-    probe keys uniform in [2^31, 2^32) with round(hit_rate * n) of them
-    replaced by build keys at random positions."""
+    """Seeded synthetic inputs on the GPU: m/12 distinct build keys in
+    [0, 2^31), probe keys distinct in [2^31, 2^32) with round(hit_rate * n) of
+    them replaced (at random positions) by random build keys."""
     n_build = (1 << log2_m) // 12
     g = torch.Generator(device="cuda").manual_seed(seed)
-    build = torch.randint(0, 2**31, (n_build,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
-    probe = torch.randint(-(2**31), 0, (n_probe,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+    build = distinct_keys(torch, n_build, 2 * seed, 0)
+    probe = distinct_keys(torch, n_probe, 2 * seed + 1, 1)
     hits = int(round(hit_rate * n_probe))
     idx = torch.randperm(n_probe, device="cuda", generator=g)[:hits]
     probe[idx] = build[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
@@ -161,8 +178,8 @@ def batched_workload(torch, D, n_filters, seed, cfg=(32, 6, 16, 16, 2), bits_per
         npb = int(2 ** rng.uniform(*log2_probe))
         h = float(rng.uniform(*hit))
         desc = D.bbf_desc(wb, bits_per_key * nb, k, w, s_, z)
-        build = torch.randint(0, 2**31, (nb,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
-        probe = torch.randint(-(2**31), 0, (npb,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+        build = distinct_keys(torch, nb, 3 * f + 100000, 0)
+        probe = distinct_keys(torch, npb, 3 * f + 100001, 1)
         hits = int(round(h * npb))
         if hits:
             idx = torch.randperm(npb, device="cuda", generator=g)[:hits]

@@ -285,12 +285,13 @@ def test_full_size_1gib_properties():
     desc = D.bbf_desc(*[cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]])
     f = O.Bloom(*cfg)
     n_build = (1 << 33) // 12
+    import bench
     g = torch.Generator(device="cuda").manual_seed(1)
-    build = torch.randint(0, 2**31, (n_build,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+    build = bench.distinct_keys(torch, n_build, 5, 0)
     words = D.new_filter(desc)
     D.bbf_build(desc, words, build)
     n_probe = 1 << 28
-    probe = torch.randint(-(2**31), 0, (n_probe,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+    probe = bench.distinct_keys(torch, n_probe, 6, 1)
     hits = int(0.01 * n_probe)
     idx = torch.randperm(n_probe, device="cuda", generator=g)[:hits]
     probe[idx] = build[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
