@@ -162,43 +162,94 @@ def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, wa
             "matches": matches}
 
 
-def batched_workload(torch, D, n_filters, seed, cfg=(32, 6, 16, 16, 2), bits_per_key=10,
-                     log2_build=(10, 22), log2_probe=(12, 24), hit=(0.001, 0.30)):
-    """The batched semi-join workload: n_filters zoned blocked Bloom filters
-    (u32 words, w=16, z=2, k=6 -> MAGIC addressing at m = 10 bits/key), build
-    sizes log-uniform in 2^10..2^22 keys, per-filter probe batches log-uniform
-    in 2^12..2^24 keys with hit rates uniform in [0.1%, 30%]; seeded."""
+FIXED_SEMI_JOIN_CFG = (32, 6, 16, 16, 2)   # (word_bits, k, w, s, z): u32 words, w=16, z=2, k=6
+
+
+def semi_join_inputs(torch, n_filters, seed, log2_build=(10, 22), log2_probe=(12, 24), hit=(0.001, 0.30)):
+    """Keys of the batched semi-join workload: n_filters partitions with build
+    sizes log-uniform in 2^10..2^22 keys and probe batches log-uniform in
+    2^12..2^24 keys at hit rates uniform in [0.1%, 30%]; seeded.  Build keys
+    are regenerated on demand (build_keys) so that only probe keys stay
+    resident."""
     import numpy as np
     rng = np.random.default_rng(seed)
     g = torch.Generator(device="cuda").manual_seed(seed)
-    wb, k, w, s_, z = cfg
-    tasks, meta = [], []
+    meta, probes = [], []
     for f in range(n_filters):
         nb = int(2 ** rng.uniform(*log2_build))
         npb = int(2 ** rng.uniform(*log2_probe))
         h = float(rng.uniform(*hit))
-        desc = D.bbf_desc(wb, bits_per_key * nb, k, w, s_, z)
-        build = distinct_keys(torch, nb, 3 * f + 100000, 0)
+        build = build_keys(torch, f, nb)
         probe = distinct_keys(torch, npb, 3 * f + 100001, 1)
         hits = int(round(h * npb))
         if hits:
             idx = torch.randperm(npb, device="cuda", generator=g)[:hits]
             probe[idx] = build[torch.randint(0, nb, (hits,), device="cuda", generator=g)]
-        words = D.new_filter(desc)
-        D.bbf_build(desc, words, build)
-        tasks.append((desc, words, probe, 0))
+        probes.append(probe)
         meta.append({"n_build": nb, "n_probe": npb, "hits": hits})
     torch.cuda.synchronize()
-    return tasks, meta
+    return meta, probes
 
 
-def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0)):
-    """Plain per-filter launches of the fixed-config kernel vs one autotuned,
-    graph-captured batch plan over all filters (same filters, same lists)."""
-    log("batched: generating %d filters" % n_filters)
-    tasks, meta = batched_workload(torch, D, n_filters, seed)
+def build_keys(torch, f, nb):
+    return distinct_keys(torch, nb, 3 * f + 100000, 0)
+
+
+def build_filters(torch, D, meta, probes, choose):
+    """One filter per partition; choose(f, n_build) -> (word_bits, k, w, s, z, m)."""
+    tasks = []
+    for f, (mt, probe) in enumerate(zip(meta, probes)):
+        wb, k, w, s_, z, m = choose(f, mt["n_build"])
+        desc = D.bbf_desc(wb, m, k, w, s_, z)
+        words = D.new_filter(desc)
+        D.bbf_build(desc, words, build_keys(torch, f, mt["n_build"]))
+        tasks.append((desc, words, probe, 0))
+    torch.cuda.synchronize()
+    return tasks
+
+
+def time_graph_plan(torch, D, tasks, steps, tune_reps):
+    """Autotunes a batch plan over tasks, captures one run into a hipGraph and
+    times replays; returns (ms per replay, counts, tuned shape, groups)."""
+    plan = D.BatchPlan(tasks)
+    best = plan.tune(reps=tune_reps)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.run(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        plan.run(s)
+    graph.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    a.record(stream)
+    for _ in range(steps):
+        graph.replay()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    counts = [int(c.item()) for c in plan.counts]
+    groups = plan.group_count()
+    plan.close()
+    del graph
+    return ms, counts, best, groups
+
+
+def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), model=None, tune_reps=3):
+    """The batched semi-join: plain per-filter launches of the fixed-config
+    kernel vs one autotuned, graph-captured batch plan over all filters (same
+    filters, same lists); with a trained cost model, also the learned form:
+    each filter's (config, m) chosen by the model for its n and tw, all
+    filters in one mixed-configuration plan.  Total filter cost = probe time
+    + tw x false positives (measured: matches - true hits)."""
+    log("batched: generating %d partitions" % n_filters)
+    meta, probes = semi_join_inputs(torch, n_filters, seed)
     n_keys = sum(m["n_probe"] for m in meta)
     true_hits = sum(m["hits"] for m in meta)
+    wb, k, w, s_, z = FIXED_SEMI_JOIN_CFG
+    tasks = build_filters(torch, D, meta, probes, lambda f, nb: (wb, k, w, s_, z, 10 * nb))
     outs = [torch.empty(max(t[2].numel(), 1), dtype=torch.int32, device="cuda") for t in tasks]
     cnts = [torch.zeros(1, dtype=torch.int64, device="cuda") for _ in tasks]
     stream = torch.cuda.current_stream()
@@ -216,45 +267,109 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0)):
     b.record(stream)
     torch.cuda.synchronize()
     plain_ms = a.elapsed_time(b) / steps
-    plain_matches = sum(int(c.item()) for c in cnts)
+    plain_counts = [int(c.item()) for c in cnts]
+    plain_matches = sum(plain_counts)
+    del outs, cnts
+    torch.cuda.empty_cache()
 
     log("batched: autotuning the batch plan")
-    plan = D.BatchPlan(tasks)
-    best = plan.tune(reps=3)
-    s = torch.cuda.Stream()
-    with torch.cuda.stream(s):
-        plan.run(s)
-    torch.cuda.synchronize()
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph, stream=s):
-        plan.run(s)
-    graph.replay()
-    torch.cuda.synchronize()
-    a.record(stream)
-    for _ in range(steps):
-        graph.replay()
-    b.record(stream)
-    torch.cuda.synchronize()
-    batch_ms = a.elapsed_time(b) / steps
-    batch_matches = sum(int(c.item()) for c in plan.counts)
-    same = all(torch.equal(c1, c2) for c1, c2 in zip(cnts, plan.counts))
-    plan.close()
+    batch_ms, counts, best, _ = time_graph_plan(torch, D, tasks, steps, tune_reps)
+    filter_bytes = int(sum(t[0].size_bytes for t in tasks))
+    del tasks
+    torch.cuda.empty_cache()
     fp = plain_matches - true_hits
-    res = {"filters": n_filters, "probe_keys": n_keys,
-           "filter_bytes": int(sum(t[0].size_bytes for t in tasks)),
+    res = {"filters": n_filters, "probe_keys": n_keys, "true_hits": true_hits,
+           "fixed_config": "u32 words, w=16, z=2, k=6, m=10 bits/key (MAGIC)",
+           "filter_bytes": filter_bytes,
            "plain_ms": round(plain_ms, 3), "batched_graph_ms": round(batch_ms, 3),
            "plain_G_probes_per_s": round(n_keys / plain_ms / 1e6, 2),
            "batched_G_probes_per_s": round(n_keys / batch_ms / 1e6, 2),
            "speedup": round(plain_ms / batch_ms, 3),
-           "matches": plain_matches, "false_positives": fp, "counts_identical": bool(same and batch_matches == plain_matches),
+           "matches": plain_matches, "false_positives": fp,
+           "counts_identical": bool(counts == plain_counts),
            "tuned": {"lds_max_bytes": best.lds_max_bytes, "block_threads": best.block_threads,
                      "chunk_tiles": best.chunk_tiles},
-           "total_filter_cost_ms": {("tw=%gns" % t): {"plain": round(plain_ms + t * fp * 1e-6, 3),
-                                                      "batched": round(batch_ms + t * fp * 1e-6, 3)}
-                                    for t in tw_ns}}
-    del tasks, outs, cnts
+           "total_filter_cost_ms": {}}
+    for t in tw_ns:
+        row = {"plain": round(plain_ms + t * fp * 1e-6, 3), "batched": round(batch_ms + t * fp * 1e-6, 3)}
+        if model is not None:
+            log("batched: learned configurations for tw=%g ns" % t)
+            choices = {}
+
+            def learned(f, nb):
+                ch = choices.get(nb)
+                if ch is None:
+                    ch = choices[nb] = model.choose(nb, t, use_mlp=True)
+                return ch.word_bits, ch.k, ch.w, ch.s, ch.z, ch.m
+
+            lt = build_filters(torch, D, meta, probes, learned)
+            l_ms, l_counts, _, groups = time_graph_plan(torch, D, lt, steps, tune_reps)
+            l_fp = sum(l_counts) - true_hits
+            used = {}
+            for desc, _, _, _ in lt:
+                key = "u%d w%d s%d z%d k%d" % (desc.word_bits, desc.w, desc.s, desc.z, desc.k)
+                used[key] = used.get(key, 0) + 1
+            l_bytes = int(sum(x[0].size_bytes for x in lt))
+            del lt
+            torch.cuda.empty_cache()
+            row["learned_batched"] = round(l_ms + t * l_fp * 1e-6, 3)
+            row["learned_probe_ms"] = round(l_ms, 3)
+            row["learned_false_positives"] = l_fp
+            row["learned_filter_bytes"] = l_bytes
+            row["learned_config_groups"] = groups
+            row["learned_configs"] = dict(sorted(used.items(), key=lambda kv: -kv[1])[:6])
+            row["plain_over_learned_batched"] = round(row["plain"] / row["learned_batched"], 3)
+        res["total_filter_cost_ms"]["tw=%gns" % t] = row
+    del probes
     torch.cuda.empty_cache()
     return res
+
+
+def run_learned(torch, D, train_fraction=0.25, samples=400000, epochs=40, model_path=None):
+    """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
+    bits, timed on the GPU; FPR measured per configuration and bits per key),
+    MLP trained on a seeded `train_fraction` of the (configuration, size)
+    cells, then the learned choice against the exhaustive optimum over
+    n = 2^10..2^28 and tw = 1..1000 ns per qualifying tuple."""
+    import numpy as np
+    log("learned: calibration sweep")
+    M = D.CostModel.calibrate()
+    calib_s = M.calibration_seconds()
+    log("learned: calibrated %d configurations in %.1f s; training" % (M.config_count(), calib_s))
+    t0 = time.time()
+    M.train(train_fraction=train_fraction, samples=samples, epochs=epochs, seed=1)
+    train_s = time.time() - t0
+    regrets, rows = [], []
+    t_ml = t_ex = 0.0
+    tws = (1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 200.0, 500.0, 1000.0)
+    for ln in range(10, 29, 2):
+        for tw in tws:
+            t0 = time.time()
+            ml = M.choose(1 << ln, tw, use_mlp=True)
+            t1 = time.time()
+            ex = M.choose(1 << ln, tw, use_mlp=False)
+            t2 = time.time()
+            t_ml += t1 - t0
+            t_ex += t2 - t1
+            r = ml.cost_ns / ex.cost_ns - 1.0
+            regrets.append(r)
+            if tw in (1.0, 100.0) and ln in (10, 20, 28):
+                rows.append({"n": 1 << ln, "tw_ns": tw, "learned": ml.as_dict(), "exhaustive": ex.as_dict(),
+                             "regret": round(r, 4)})
+    regrets = np.array(regrets)
+    if model_path:
+        M.save(model_path)
+    res = {"configurations": M.config_count(), "filter_sizes_log2": [13, 33],
+           "calibration_s": round(calib_s, 2), "train_s": round(train_s, 2),
+           "train_fraction_of_cells": train_fraction, "train_samples": samples,
+           "queries": int(regrets.size), "regret_mean": round(float(regrets.mean()), 4),
+           "regret_p95": round(float(np.percentile(regrets, 95)), 4),
+           "regret_max": round(float(regrets.max()), 4),
+           "share_within_5pct": round(float((regrets <= 0.05).mean()), 3),
+           "choose_ms_learned": round(1e3 * t_ml / regrets.size, 3),
+           "choose_ms_exhaustive_table": round(1e3 * t_ex / regrets.size, 3),
+           "examples": rows}
+    return res, M
 
 
 def load_traffic(name):
@@ -280,6 +395,11 @@ def main():
     ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
     ap.add_argument("--batched", type=int, default=0, metavar="FILTERS",
                     help="only run the batched semi-join workload with this many filters (4096 = full)")
+    ap.add_argument("--learned", action="store_true",
+                    help="only run the learned configuration model (calibration, training, regret); with "
+                         "--batched also the learned + batched semi-join")
+    ap.add_argument("--train-fraction", type=float, default=0.25)
+    ap.add_argument("--model-path", default=None, help="save the calibrated + trained model here")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -293,6 +413,15 @@ def main():
     D.load()
     torch.cuda.set_device(0)
 
+    if args.learned:
+        res, model = run_learned(torch, D, train_fraction=args.train_fraction, model_path=args.model_path)
+        print(json.dumps({"learned_config": res}), flush=True)
+        if args.batched:
+            b = run_batched(torch, D, args.batched, max(2, args.steps // 4), tw_ns=(1.0, 10.0, 100.0, 1000.0),
+                            model=model, tune_reps=1)
+            print(json.dumps({"batched_semi_join": b}), flush=True)
+        model.close()
+        return
     if args.batched:
         print(json.dumps({"batched_semi_join": run_batched(torch, D, args.batched, max(2, args.steps // 4))}),
               flush=True)

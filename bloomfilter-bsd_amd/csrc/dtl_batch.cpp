@@ -13,9 +13,9 @@
 
 using namespace dtlh;
 
-struct dtl_batch_plan {
+// One configuration's share of a plan: its tasks, tile table and scratch.
+struct PlanGroup {
   const BbfEntry* entry = nullptr;
-  int device = 0;
   uint32_t n_tasks = 0;
   uint32_t total_tiles = 0;
   BatchTaskDev* d_tasks = nullptr;
@@ -23,38 +23,46 @@ struct dtl_batch_plan {
   uint64_t* d_bits = nullptr;
   uint32_t* d_tile_cnt = nullptr;  // total_tiles counts + 1 work counter
   BbfBatchLaunch launch{};
+  uint64_t keys = 0;               // probe keys in the group (largest group first)
 };
 
-static void plan_free(dtl_batch_plan* p) {
+struct dtl_batch_plan {
+  int device = 0;
+  std::vector<PlanGroup> groups;   // one per distinct (word_bits, w, s, z, k)
+};
+
+namespace {
+
+void group_free(PlanGroup& g) {
+  if (g.d_tasks) (void)hipFree(g.d_tasks);
+  if (g.d_tile_task) (void)hipFree(g.d_tile_task);
+  if (g.d_bits) (void)hipFree(g.d_bits);
+  if (g.d_tile_cnt) (void)hipFree(g.d_tile_cnt);
+  g = PlanGroup{};
+}
+
+void plan_free(dtl_batch_plan* p) {
   if (!p) return;
-  if (p->d_tasks) (void)hipFree(p->d_tasks);
-  if (p->d_tile_task) (void)hipFree(p->d_tile_task);
-  if (p->d_bits) (void)hipFree(p->d_bits);
-  if (p->d_tile_cnt) (void)hipFree(p->d_tile_cnt);
+  for (PlanGroup& g : p->groups) group_free(g);
   delete p;
 }
 
-extern "C" {
+bool same_config(const dtl_bbf_desc& a, const dtl_bbf_desc& b) {
+  return a.word_bits == b.word_bits && a.w == b.w && a.s == b.s && a.z == b.z && a.k == b.k;
+}
 
-int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
-                          const dtl_probe_params* params, dtl_batch_plan** out) {
-  if (!out || !tasks || n_tasks == 0) return DTL_EINVAL;
-  *out = nullptr;
-  const dtl_bbf_desc& d0 = tasks[0].desc;
+// Builds one group from tasks that share a configuration.
+int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, const dtl_probe_params* params,
+                 uint32_t cus, uint32_t max_lds, PlanGroup& g) {
+  const dtl_bbf_desc& d0 = tasks[ids[0]].desc;
   const BbfEntry* e = bbf_find(d0.word_bits, d0.w, d0.s, d0.z, d0.k);
   if (!e || !e->probe_batched) return DTL_EINVAL;
-  int dev = 0, cus = 0;
-  uint32_t max_lds = 0;
-  int rc = current_device(&dev, &cus, &max_lds);
-  if (rc != DTL_OK) return rc;
-
+  const uint32_t n_tasks = static_cast<uint32_t>(ids.size());
   std::vector<BatchTaskDev> host(n_tasks);
-  uint64_t tiles = 0;
+  uint64_t tiles = 0, keys = 0;
   for (uint32_t t = 0; t < n_tasks; ++t) {
-    const dtl_bbf_task& T = tasks[t];
+    const dtl_bbf_task& T = tasks[ids[t]];
     const dtl_bbf_desc& d = T.desc;
-    if (d.word_bits != d0.word_bits || d.w != d0.w || d.s != d0.s || d.z != d0.z || d.k != d0.k)
-      return DTL_EINVAL;
     if (d.block_cnt < 2 || !T.d_words || !T.d_count || T.n > 0xFFFFFFFFull) return DTL_EINVAL;
     if (T.n && (!T.d_keys || !T.d_pos)) return DTL_EINVAL;
     const uint32_t bb = d.w * d.word_bits / 8;
@@ -81,6 +89,7 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
     }
     h.filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d.size_bytes, 0xFFFFFFFFull));
     tiles += h.tile_count;
+    keys += T.n;
     if (tiles > 0xFFFFFFFFull) return DTL_EINVAL;
   }
   std::vector<uint32_t> tile_task(std::max<uint64_t>(tiles, 1), 0);
@@ -88,34 +97,28 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
     std::fill(tile_task.begin() + host[t].tile_begin,
               tile_task.begin() + host[t].tile_begin + host[t].tile_count, t);
 
-  dtl_batch_plan* p = new dtl_batch_plan();
-  p->entry = e;
-  p->device = dev;
-  p->n_tasks = n_tasks;
-  p->total_tiles = static_cast<uint32_t>(tiles);
+  g.entry = e;
+  g.n_tasks = n_tasks;
+  g.total_tiles = static_cast<uint32_t>(tiles);
+  g.keys = keys;
   const size_t nt = std::max<size_t>(tiles, 1);
-  if (hipMalloc(&p->d_tasks, sizeof(BatchTaskDev) * n_tasks) != hipSuccess ||
-      hipMalloc(&p->d_tile_task, sizeof(uint32_t) * nt) != hipSuccess ||
-      hipMalloc(&p->d_bits, nt * kTileWords * 8) != hipSuccess ||
-      hipMalloc(&p->d_tile_cnt, (nt + 1) * sizeof(uint32_t)) != hipSuccess) {
-    plan_free(p);
+  if (hipMalloc(&g.d_tasks, sizeof(BatchTaskDev) * n_tasks) != hipSuccess ||
+      hipMalloc(&g.d_tile_task, sizeof(uint32_t) * nt) != hipSuccess ||
+      hipMalloc(&g.d_bits, nt * kTileWords * 8) != hipSuccess ||
+      hipMalloc(&g.d_tile_cnt, (nt + 1) * sizeof(uint32_t)) != hipSuccess)
     return DTL_ENOMEM;
-  }
-  if (hipMemcpy(p->d_tasks, host.data(), sizeof(BatchTaskDev) * n_tasks, hipMemcpyHostToDevice) !=
+  if (hipMemcpy(g.d_tasks, host.data(), sizeof(BatchTaskDev) * n_tasks, hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(p->d_tile_task, tile_task.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice) !=
-          hipSuccess) {
-    plan_free(p);
+      hipMemcpy(g.d_tile_task, tile_task.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice) != hipSuccess)
     return DTL_EHIP;
-  }
-  BbfBatchLaunch& L = p->launch;
-  L.tasks = p->d_tasks;
-  L.tile_task = p->d_tile_task;
+  BbfBatchLaunch& L = g.launch;
+  L.tasks = g.d_tasks;
+  L.tile_task = g.d_tile_task;
   L.n_tasks = n_tasks;
-  L.total_tiles = p->total_tiles;
-  L.bits = p->d_bits;
-  L.tile_cnt = p->d_tile_cnt;
-  L.work_counter = p->d_tile_cnt + nt;
+  L.total_tiles = g.total_tiles;
+  L.bits = g.d_bits;
+  L.tile_cnt = g.d_tile_cnt;
+  L.work_counter = g.d_tile_cnt + nt;
   const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
   uint32_t lds_max = 32 * 1024;
   if (params && params->lds_max_bytes) lds_max = params->lds_max_bytes;
@@ -125,6 +128,63 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
   L.grid = params ? params->grid_blocks : 0;
   L.chunk = (params && params->chunk_tiles) ? params->chunk_tiles : 1;
   L.cu_count = cus;
+  return DTL_OK;
+}
+
+hipError_t group_run(const PlanGroup& g, hipStream_t st) {
+  BbfBatchLaunch L = g.launch;
+  L.stream = st;
+  hipError_t e = zero_u32(L.work_counter, st);
+  if (e == hipSuccess && g.total_tiles > 0) e = g.entry->probe_batched(L);
+  if (e == hipSuccess) e = compact_tasks(L.bits, L.tile_cnt, L.tasks, L.tile_task, g.n_tasks, g.total_tiles, st);
+  return e;
+}
+
+void copy_shape(const BbfBatchLaunch& L, dtl_probe_params* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->lds_max_bytes = L.lds_max_bytes;
+  out->block_threads = L.threads;
+  out->grid_blocks = L.grid;
+  out->chunk_tiles = L.chunk;
+  out->flags = L.lds_max_bytes == 0 ? DTL_PROBE_FORCE_GLOBAL : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Tasks may mix configurations: they are grouped by (word_bits, w, s, z, k),
+// in order of first appearance, and each group gets its own persistent launch;
+// every task's list is the same as bbf_probe's for it either way.
+int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
+                          const dtl_probe_params* params, dtl_batch_plan** out) {
+  if (!out || !tasks || n_tasks == 0) return DTL_EINVAL;
+  *out = nullptr;
+  int dev = 0, cus = 0;
+  uint32_t max_lds = 0;
+  int rc = current_device(&dev, &cus, &max_lds);
+  if (rc != DTL_OK) return rc;
+  std::vector<std::vector<uint32_t>> ids;
+  std::vector<uint32_t> rep;  // first task of each group
+  for (uint32_t t = 0; t < n_tasks; ++t) {
+    size_t g = 0;
+    while (g < rep.size() && !same_config(tasks[rep[g]].desc, tasks[t].desc)) ++g;
+    if (g == rep.size()) {
+      rep.push_back(t);
+      ids.emplace_back();
+    }
+    ids[g].push_back(t);
+  }
+  dtl_batch_plan* p = new dtl_batch_plan();
+  p->device = dev;
+  p->groups.resize(ids.size());
+  for (size_t g = 0; g < ids.size(); ++g) {
+    rc = group_create(tasks, ids[g], params, static_cast<uint32_t>(cus), max_lds, p->groups[g]);
+    if (rc != DTL_OK) {
+      plan_free(p);
+      return rc;
+    }
+  }
   *out = p;
   return DTL_OK;
 }
@@ -132,13 +192,9 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
 int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
   if (!p) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  BbfBatchLaunch L = p->launch;
-  L.stream = st;
   (void)hipGetLastError();  // report only this call's failures
-  hipError_t e = zero_u32(L.work_counter, st);
-  if (e == hipSuccess && p->total_tiles > 0) e = p->entry->probe_batched(L);
-  if (e == hipSuccess)
-    e = compact_tasks(L.bits, L.tile_cnt, L.tasks, L.tile_task, p->n_tasks, p->total_tiles, st);
+  hipError_t e = hipSuccess;
+  for (size_t g = 0; g < p->groups.size() && e == hipSuccess; ++g) e = group_run(p->groups[g], st);
   if (e != hipSuccess) {
     set_last_error(e);
     return DTL_EHIP;
@@ -146,13 +202,18 @@ int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
   return DTL_OK;
 }
 
+uint32_t bbf_batch_plan_group_count(const dtl_batch_plan* p) {
+  return p ? static_cast<uint32_t>(p->groups.size()) : 0u;
+}
+
 void bbf_batch_plan_destroy(dtl_batch_plan* p) { plan_free(p); }
 
-// Autotuner: times every candidate launch shape of the persistent probe
-// kernel on the plan's own tasks (median of `reps` runs, HIP events on
-// `stream`) and keeps the fastest.  Candidates: LDS staging threshold
-// {0, 8, 16, 32, 64, 96, 150 KiB} x workgroup {256, 1024} threads x tiles per
-// grab {1, 2, 4, 8, 16}; the grid is CUs x occupancy.
+// Autotuner: for each configuration group, times every candidate launch
+// shape of the persistent probe kernel on the group's own tasks (median of
+// `reps` runs, HIP events on `stream`) and keeps the fastest.  Candidates:
+// LDS staging threshold {0, 8, 16, 32, 64, 96, 150 KiB} x workgroup {256,
+// 1024} threads x tiles per grab {1, 2, 4, 8, 16}; the grid is CUs x
+// occupancy.  *best_out receives the shape of the group with the most keys.
 int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
                         dtl_probe_params* best_out) {
   if (!p || reps == 0) return DTL_EINVAL;
@@ -161,55 +222,58 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
   uint32_t max_lds = 0;
   int rc = current_device(&dev, &cus, &max_lds);
   if (rc != DTL_OK) return rc;
+  (void)hipGetLastError();
   const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
   const uint32_t lds_opts[] = {0, 8 << 10, 16 << 10, 32 << 10, 64 << 10, 96 << 10, 150 << 10};
   const uint32_t thr_opts[] = {256, 1024};
   const uint32_t chunk_opts[] = {1, 2, 4, 8, 16};
   hipEvent_t a = nullptr, b = nullptr;
-  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return DTL_EHIP;
-  const BbfBatchLaunch saved = p->launch;
-  float best_ms = 1e30f;
-  BbfBatchLaunch best = saved;
-  rc = DTL_OK;
-  for (uint32_t lds : lds_opts) {
-    if (lds > lds_cap) continue;
-    for (uint32_t thr : thr_opts) {
-      for (uint32_t ch : chunk_opts) {
-        p->launch.lds_max_bytes = lds;
-        p->launch.threads = thr;
-        p->launch.chunk = ch;
-        p->launch.grid = 0;
-        std::vector<float> t;
-        for (uint32_t r = 0; r < reps + 1 && rc == DTL_OK; ++r) {
-          if (hipEventRecord(a, st) != hipSuccess) rc = DTL_EHIP;
-          if (rc == DTL_OK) rc = bbf_batch_plan_run(p, stream);
-          if (rc == DTL_OK && hipEventRecord(b, st) != hipSuccess) rc = DTL_EHIP;
-          if (rc == DTL_OK && hipEventSynchronize(b) != hipSuccess) rc = DTL_EHIP;
-          float ms = 0;
-          if (rc == DTL_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = DTL_EHIP;
-          if (r > 0) t.push_back(ms);  // the first run warms up
-        }
-        if (rc != DTL_OK) break;
-        std::sort(t.begin(), t.end());
-        const float med = t[t.size() / 2];
-        if (med < best_ms) {
-          best_ms = med;
-          best = p->launch;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    return DTL_EHIP;
+  }
+  size_t largest = 0;
+  for (size_t gi = 0; gi < p->groups.size() && rc == DTL_OK; ++gi) {
+    PlanGroup& g = p->groups[gi];
+    if (g.keys > p->groups[largest].keys) largest = gi;
+    const BbfBatchLaunch saved = g.launch;
+    float best_ms = 1e30f;
+    BbfBatchLaunch best = saved;
+    for (uint32_t lds : lds_opts) {
+      if (lds > lds_cap || rc != DTL_OK) continue;
+      for (uint32_t thr : thr_opts) {
+        for (uint32_t ch : chunk_opts) {
+          g.launch.lds_max_bytes = lds;
+          g.launch.threads = thr;
+          g.launch.chunk = ch;
+          g.launch.grid = 0;
+          std::vector<float> t;
+          for (uint32_t r = 0; r < reps + 1 && rc == DTL_OK; ++r) {
+            if (hipEventRecord(a, st) != hipSuccess || group_run(g, st) != hipSuccess ||
+                hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+              rc = DTL_EHIP;
+              break;
+            }
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = DTL_EHIP;
+            if (r > 0) t.push_back(ms);  // the first run warms up
+          }
+          if (rc != DTL_OK) break;
+          std::sort(t.begin(), t.end());
+          const float med = t[t.size() / 2];
+          if (med < best_ms) {
+            best_ms = med;
+            best = g.launch;
+          }
         }
       }
     }
+    g.launch = (rc == DTL_OK) ? best : saved;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  p->launch = (rc == DTL_OK) ? best : saved;
-  if (best_out) {
-    std::memset(best_out, 0, sizeof(*best_out));
-    best_out->lds_max_bytes = p->launch.lds_max_bytes;
-    best_out->block_threads = p->launch.threads;
-    best_out->grid_blocks = p->launch.grid;
-    best_out->chunk_tiles = p->launch.chunk;
-    best_out->flags = p->launch.lds_max_bytes == 0 ? DTL_PROBE_FORCE_GLOBAL : 0;
-  }
+  if (rc == DTL_EHIP) set_last_error(hipGetLastError());
+  if (best_out && !p->groups.empty()) copy_shape(p->groups[largest].launch, best_out);
   return rc;
 }
 

@@ -1,0 +1,467 @@
+// dtl_model.cpp -- choosing a blocked Bloom filter configuration from (n, tw):
+// a GPU calibration sweep, an FP32 MLP trained on part of it, and the
+// exhaustive table as fallback and judge.
+//
+// Objective (the reference's, cost_fn.hpp:143-166):
+//   overhead(config, m, n, tw) = t_lookup(config, m) + tw * fpr(config, m, n)
+// per probe, in nanoseconds here (tw = downstream work per qualifying tuple).
+//   * t_lookup: measured GPU probe time per key (whole bbf_probe call, 2^24
+//     keys, zero filter) at filter sizes 2^13 .. 2^33 bits, interpolated in
+//     log2(m) -- the GPU analogue of the reference's per-memory-level timings
+//     (model/calibration.cpp:251-311, benchmark.cpp:104-199);
+//   * fpr: measured on the GPU for every configuration at 8 bits-per-key
+//     points (n = 2^16 built keys, 2^22 absent keys probed), interpolated in
+//     log(fpr) over bits per key.
+// Candidate space: the reference's valid configurations (model/util.hpp:16-63:
+// word size x w <= 64 B, s <= w, k % s == 0; zoned w >= 4) with z in {1,2,4},
+// and m in [4n, 20n] bits like the reference's optimizer range
+// (skyline_matrix_builder.cpp:170-198).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dtl_hip.h"
+#include "dtl_internal.hpp"
+
+using namespace dtlh;
+
+namespace {
+
+struct Cfg {
+  uint32_t word_bits, w, s, z, k;
+};
+
+const double kBpk[] = {4, 6, 8, 10, 12, 16, 20, 24};
+constexpr int kNBpk = 8;
+
+std::vector<Cfg> candidate_configs() {
+  std::vector<Cfg> out;
+  for (uint32_t k = 1; k <= 16; ++k)
+    for (uint32_t ws : {4u, 8u})
+      for (uint32_t w : {1u, 2u, 4u, 8u, 16u}) {
+        if (ws * w > 64) continue;
+        for (uint32_t s : {1u, 2u, 4u, 8u, 16u}) {
+          if (s > w || s > k || k % s) continue;
+          out.push_back({ws * 8, w, s, 1, k});
+        }
+        if (w >= 4)
+          for (uint32_t z : {2u, 4u}) {
+            if (z >= w || z > k || k % z || (k & 1u)) continue;
+            out.push_back({ws * 8, w, w, z, k});
+          }
+      }
+  return out;
+}
+
+}  // namespace
+
+struct dtl_cost_model {
+  std::vector<Cfg> cfgs;
+  uint32_t log2_min = 13, log2_max = 33;
+  std::vector<double> t_ns;   // [cfg][size] ns per probe
+  std::vector<double> fpr;    // [cfg][bpk]
+  std::vector<uint8_t> train_cell;  // [cfg][size] 1 if the MLP may learn from it
+  dtl_mlp* mlp = nullptr;
+  double y_mean = 0, y_std = 1;
+  double calib_seconds = 0;
+  int nsizes() const { return static_cast<int>(log2_max - log2_min + 1); }
+};
+
+namespace {
+
+double interp_t(const dtl_cost_model& M, int c, double log2m) {
+  const int ns = M.nsizes();
+  double x = log2m - M.log2_min;
+  if (x <= 0) return M.t_ns[c * ns];
+  if (x >= ns - 1) return M.t_ns[c * ns + ns - 1];
+  const int i = static_cast<int>(x);
+  const double f = x - i;
+  return M.t_ns[c * ns + i] * (1 - f) + M.t_ns[c * ns + i + 1] * f;
+}
+
+double interp_fpr(const dtl_cost_model& M, int c, double bpk) {
+  const double* F = &M.fpr[static_cast<size_t>(c) * kNBpk];
+  auto lg = [](double v) { return std::log(std::max(v, 1e-9)); };
+  if (bpk <= kBpk[0]) {
+    const double s = (lg(F[1]) - lg(F[0])) / (kBpk[1] - kBpk[0]);
+    return std::min(1.0, std::exp(lg(F[0]) + s * (bpk - kBpk[0])));
+  }
+  for (int i = 0; i + 1 < kNBpk; ++i)
+    if (bpk <= kBpk[i + 1]) {
+      const double f = (bpk - kBpk[i]) / (kBpk[i + 1] - kBpk[i]);
+      return std::exp(lg(F[i]) * (1 - f) + lg(F[i + 1]) * f);
+    }
+  const double s = (lg(F[kNBpk - 1]) - lg(F[kNBpk - 2])) / (kBpk[kNBpk - 1] - kBpk[kNBpk - 2]);
+  return std::exp(lg(F[kNBpk - 1]) + s * (bpk - kBpk[kNBpk - 1]));
+}
+
+double table_cost(const dtl_cost_model& M, int c, double m, double n, double tw) {
+  return interp_t(M, c, std::log2(m)) + tw * interp_fpr(M, c, m / n);
+}
+
+void features(const Cfg& c, double m, double n, double tw, float* f) {
+  f[0] = c.word_bits / 64.0f;
+  f[1] = static_cast<float>(std::log2(c.w) / 4.0);
+  f[2] = static_cast<float>(std::log2(c.s) / 4.0);
+  f[3] = static_cast<float>(std::log2(c.z) / 2.0);
+  f[4] = c.k / 16.0f;
+  f[5] = static_cast<float>((std::log2(m) - 23.0) / 10.0);
+  f[6] = static_cast<float>((std::log2(n) - 19.0) / 9.0);
+  f[7] = static_cast<float>(std::log2(tw) / 10.0);
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  bool alloc(size_t bytes) { return hipMalloc(&p, bytes) == hipSuccess; }
+};
+
+uint64_t read_u64(const void* d, hipStream_t st) {
+  uint64_t v = 0;
+  (void)hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, st);
+  (void)hipStreamSynchronize(st);
+  return v;
+}
+
+// Candidate m values for a query: n * {4..20} bits, within the calibrated range.
+std::vector<double> candidate_m(const dtl_cost_model& M, const Cfg& c, uint64_t n) {
+  std::vector<double> out;
+  const double lo = std::ldexp(1.0, M.log2_min), hi = std::ldexp(1.0, M.log2_max);
+  const double min_m = 2.0 * c.word_bits * c.w + 1;
+  for (int b = 4; b <= 20; ++b) {
+    const double m = std::min(hi, std::max({lo, min_m, static_cast<double>(n) * b}));
+    if (out.empty() || out.back() != m) out.push_back(m);
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_stream_t stream) {
+  if (!out) return DTL_EINVAL;
+  *out = nullptr;
+  const uint32_t lmin = (p && p->log2_m_min) ? p->log2_m_min : 13;
+  const uint32_t lmax = (p && p->log2_m_max) ? p->log2_m_max : 33;
+  const uint32_t lkeys = (p && p->probe_keys_log2) ? p->probe_keys_log2 : 24;
+  const uint32_t reps = (p && p->reps) ? p->reps : 3;
+  const uint32_t lfpr = (p && p->fpr_keys_log2) ? p->fpr_keys_log2 : 22;
+  if (lmin < 10 || lmax > 34 || lmin > lmax || lkeys > 30) return DTL_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const auto t0 = std::chrono::steady_clock::now();
+  dtl_cost_model* M = new dtl_cost_model();
+  M->cfgs = candidate_configs();
+  M->log2_min = lmin;
+  M->log2_max = lmax;
+  const int nc = static_cast<int>(M->cfgs.size()), ns = M->nsizes();
+  M->t_ns.assign(static_cast<size_t>(nc) * ns, 0.0);
+  M->fpr.assign(static_cast<size_t>(nc) * kNBpk, 0.0);
+  M->train_cell.assign(static_cast<size_t>(nc) * ns, 1);
+
+  const uint64_t nkeys = 1ull << lkeys, nfpr = 1ull << lfpr, nbuild = 1ull << 16;
+  const uint64_t filter_bytes = (1ull << lmax) / 8 + 4096;
+  const uint64_t max_fbytes = static_cast<uint64_t>(kBpk[kNBpk - 1] * nbuild / 8) + 4096;
+  DevBuf filt, keys, pos, cnt, bkeys, absent, fw;
+  if (!filt.alloc(filter_bytes) || !keys.alloc(nkeys * 4) || !pos.alloc(std::max(nkeys, nfpr) * 4) ||
+      !cnt.alloc(8) || !bkeys.alloc(nbuild * 4) || !absent.alloc(nfpr * 4) || !fw.alloc(max_fbytes)) {
+    delete M;
+    return DTL_ENOMEM;
+  }
+  int rc = DTL_OK;
+  if (hipMemsetAsync(filt.p, 0, filter_bytes, st) != hipSuccess ||
+      gen_keys(static_cast<uint32_t*>(keys.p), nkeys, 977, 1, st) != hipSuccess)
+    rc = DTL_EHIP;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (rc == DTL_OK && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = DTL_EHIP;
+  // lookup cost per (config, size): a zero filter, so every key is a
+  // negative probe (the common case) and the time covers the whole call
+  for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+    const Cfg& C = M->cfgs[c];
+    for (int si = 0; si < ns && rc == DTL_OK; ++si) {
+      // a non-power-of-two size (MAGIC addressing), as chosen sizes m = b*n are
+      const uint64_t m = (1ull << (lmin + si)) - 2ull * C.word_bits * C.w;
+      dtl_bbf_desc d;
+      rc = dtl_bbf_desc_init(&d, C.word_bits, std::max<uint64_t>(m, 3ull * C.word_bits * C.w), C.k, C.w,
+                             C.s, C.z);
+      if (rc != DTL_OK) break;
+      std::vector<float> ts;
+      for (uint32_t r = 0; r <= reps && rc == DTL_OK; ++r) {
+        if (hipEventRecord(a, st) != hipSuccess) rc = DTL_EHIP;
+        if (rc == DTL_OK)
+          rc = bbf_probe(&d, filt.p, static_cast<uint32_t*>(keys.p), nkeys, 0, static_cast<uint32_t*>(pos.p),
+                         static_cast<uint64_t*>(cnt.p), stream);
+        if (rc == DTL_OK && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess))
+          rc = DTL_EHIP;
+        float ms = 0;
+        if (rc == DTL_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = DTL_EHIP;
+        if (r > 0) ts.push_back(ms);
+      }
+      if (rc != DTL_OK) break;
+      std::sort(ts.begin(), ts.end());
+      M->t_ns[static_cast<size_t>(c) * ns + si] = ts[ts.size() / 2] * 1e6 / static_cast<double>(nkeys);
+    }
+  }
+  // false-positive rate per (config, bits per key), measured on real filters
+  if (rc == DTL_OK && gen_keys(static_cast<uint32_t*>(absent.p), nfpr, 4242, 1, st) != hipSuccess) rc = DTL_EHIP;
+  for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+    const Cfg& C = M->cfgs[c];
+    for (int bi = 0; bi < kNBpk && rc == DTL_OK; ++bi) {
+      dtl_bbf_desc d;
+      rc = dtl_bbf_desc_init(&d, C.word_bits, static_cast<uint64_t>(kBpk[bi] * nbuild) + 1, C.k, C.w, C.s, C.z);
+      if (rc != DTL_OK) break;
+      if (hipMemsetAsync(fw.p, 0, d.size_bytes, st) != hipSuccess ||
+          gen_keys(static_cast<uint32_t*>(bkeys.p), nbuild, 1000 + c * 16 + bi, 0, st) != hipSuccess) {
+        rc = DTL_EHIP;
+        break;
+      }
+      rc = bbf_build(&d, fw.p, static_cast<uint32_t*>(bkeys.p), nbuild, stream);
+      if (rc == DTL_OK)
+        rc = bbf_probe(&d, fw.p, static_cast<uint32_t*>(absent.p), nfpr, 0, static_cast<uint32_t*>(pos.p),
+                       static_cast<uint64_t*>(cnt.p), stream);
+      if (rc != DTL_OK) break;
+      const uint64_t fp = read_u64(cnt.p, st);
+      M->fpr[static_cast<size_t>(c) * kNBpk + bi] = std::max(1.0, static_cast<double>(fp)) / nfpr;
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (rc != DTL_OK) {
+    delete M;
+    return rc;
+  }
+  M->calib_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = M;
+  return DTL_OK;
+}
+
+void dtl_model_destroy(dtl_cost_model* M) {
+  if (!M) return;
+  if (M->mlp) dtl_mlp_destroy(M->mlp);
+  delete M;
+}
+
+int dtl_model_config_count(const dtl_cost_model* M) { return M ? static_cast<int>(M->cfgs.size()) : 0; }
+
+double dtl_model_calibration_seconds(const dtl_cost_model* M) { return M ? M->calib_seconds : 0.0; }
+
+int dtl_model_get_config(const dtl_cost_model* M, int i, dtl_bbf_choice* out) {
+  if (!M || !out || i < 0 || i >= static_cast<int>(M->cfgs.size())) return DTL_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  const Cfg& c = M->cfgs[i];
+  out->word_bits = c.word_bits;
+  out->w = c.w;
+  out->s = c.s;
+  out->z = c.z;
+  out->k = c.k;
+  return DTL_OK;
+}
+
+// Measured lookup time (ns per probe) and FPR of a configuration.
+double dtl_model_lookup_ns(const dtl_cost_model* M, int cfg, double m) {
+  return (M && cfg >= 0 && cfg < static_cast<int>(M->cfgs.size())) ? interp_t(*M, cfg, std::log2(m)) : -1.0;
+}
+
+double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
+  return (M && cfg >= 0 && cfg < static_cast<int>(M->cfgs.size())) ? interp_fpr(*M, cfg, bits_per_key) : -1.0;
+}
+
+// Trains the MLP on a seeded `train_fraction` of the calibrated (config,
+// size) cells: `samples` rows, each a random cell with bits per key
+// log-uniform in [4, 24] and tw log-uniform in [1, 1000] ns, target
+// log(overhead) standardised.
+int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, uint32_t epochs,
+                    uint32_t seed, dtl_stream_t stream) {
+  if (!M || samples == 0 || train_fraction <= 0 || train_fraction > 1) return DTL_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nc = static_cast<int>(M->cfgs.size()), ns = M->nsizes();
+  uint64_t s = 0x853C49E6748FEA9Bull ^ (static_cast<uint64_t>(seed) * 0x9E3779B97F4A7C15ull);
+  auto next = [&]() {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return s;
+  };
+  auto unif = [&]() { return (next() >> 11) * (1.0 / 9007199254740992.0); };
+  std::vector<int> cells;
+  for (int i = 0; i < nc * ns; ++i) {
+    M->train_cell[i] = unif() < train_fraction ? 1 : 0;
+    if (M->train_cell[i]) cells.push_back(i);
+  }
+  if (cells.empty()) return DTL_EINVAL;
+  std::vector<float> X(static_cast<size_t>(samples) * 8), Y(samples);
+  double sum = 0, sum2 = 0;
+  for (uint32_t i = 0; i < samples; ++i) {
+    const int cell = cells[next() % cells.size()];
+    const int c = cell / ns, si = cell % ns;
+    const double m = std::ldexp(1.0, M->log2_min + si);
+    const double bpk = 4.0 * std::pow(6.0, unif());
+    const double tw = std::pow(1000.0, unif());
+    const double n = m / bpk;
+    const double y = std::log(M->t_ns[cell] + tw * interp_fpr(*M, c, bpk));
+    features(M->cfgs[c], m, n, tw, &X[static_cast<size_t>(i) * 8]);
+    Y[i] = static_cast<float>(y);
+    sum += y;
+    sum2 += y * y;
+  }
+  M->y_mean = sum / samples;
+  M->y_std = std::sqrt(std::max(1e-12, sum2 / samples - M->y_mean * M->y_mean));
+  for (auto& y : Y) y = static_cast<float>((y - M->y_mean) / M->y_std);
+  DevBuf dx, dy;
+  if (!dx.alloc(X.size() * 4) || !dy.alloc(Y.size() * 4)) return DTL_ENOMEM;
+  if (hipMemcpyAsync(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dy.p, Y.data(), Y.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return DTL_EHIP;
+  if (M->mlp) dtl_mlp_destroy(M->mlp);
+  M->mlp = nullptr;
+  int rc = dtl_mlp_create(seed, &M->mlp);
+  if (rc != DTL_OK) return rc;
+  const uint32_t ep = epochs ? epochs : 40;
+  // learning-rate schedule: 3 phases
+  rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 2 + 1, 512,
+                     2e-3f, seed, stream);
+  if (rc == DTL_OK)
+    rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 3 + 1,
+                       1024, 5e-4f, seed + 1, stream);
+  if (rc == DTL_OK)
+    rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 6 + 1,
+                       2048, 1e-4f, seed + 2, stream);
+  if (rc == DTL_OK && hipStreamSynchronize(st) != hipSuccess) rc = DTL_EHIP;
+  return rc;
+}
+
+// Best (config, m) for n keys and tw ns per qualifying tuple.  use_mlp = 1:
+// argmin of the MLP's prediction over every candidate (one GPU inference
+// launch); use_mlp = 0: argmin of the calibrated table (the exhaustive
+// search).  out->cost_ns is always the table cost of the choice.
+int dtl_model_choose(const dtl_cost_model* M, uint64_t n, double tw_ns, int use_mlp, dtl_bbf_choice* out,
+                     dtl_stream_t stream) {
+  if (!M || !out || n == 0 || tw_ns <= 0) return DTL_EINVAL;
+  if (use_mlp && !M->mlp) return DTL_EINVAL;
+  const int nc = static_cast<int>(M->cfgs.size());
+  std::vector<int> cand_c;
+  std::vector<double> cand_m;
+  for (int c = 0; c < nc; ++c)
+    for (double m : candidate_m(*M, M->cfgs[c], n)) {
+      cand_c.push_back(c);
+      cand_m.push_back(m);
+    }
+  const size_t nk = cand_c.size();
+  std::vector<double> score(nk);
+  if (use_mlp) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<float> X(nk * 8), Y(nk);
+    for (size_t i = 0; i < nk; ++i) features(M->cfgs[cand_c[i]], cand_m[i], static_cast<double>(n), tw_ns, &X[i * 8]);
+    DevBuf dx, dy;
+    if (!dx.alloc(X.size() * 4) || !dy.alloc(nk * 4)) return DTL_ENOMEM;
+    if (hipMemcpyAsync(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) return DTL_EHIP;
+    const int rc = dtl_mlp_predict(M->mlp, static_cast<float*>(dx.p), static_cast<uint32_t>(nk),
+                                   static_cast<float*>(dy.p), stream);
+    if (rc != DTL_OK) return rc;
+    if (hipMemcpyAsync(Y.data(), dy.p, nk * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return DTL_EHIP;
+    for (size_t i = 0; i < nk; ++i) score[i] = Y[i];
+  } else {
+    for (size_t i = 0; i < nk; ++i) score[i] = table_cost(*M, cand_c[i], cand_m[i], static_cast<double>(n), tw_ns);
+  }
+  const size_t best = static_cast<size_t>(std::min_element(score.begin(), score.end()) - score.begin());
+  const Cfg& C = M->cfgs[cand_c[best]];
+  out->word_bits = C.word_bits;
+  out->w = C.w;
+  out->s = C.s;
+  out->z = C.z;
+  out->k = C.k;
+  out->m = static_cast<uint64_t>(cand_m[best]);
+  out->cost_ns = table_cost(*M, cand_c[best], cand_m[best], static_cast<double>(n), tw_ns);
+  out->predicted = use_mlp ? std::exp(score[best] * M->y_std + M->y_mean) : score[best];
+  return DTL_OK;
+}
+
+// Table cost (ns per probe) of a given choice.
+double dtl_model_cost(const dtl_cost_model* M, const dtl_bbf_choice* ch, uint64_t n, double tw_ns) {
+  if (!M || !ch) return -1.0;
+  for (size_t c = 0; c < M->cfgs.size(); ++c) {
+    const Cfg& C = M->cfgs[c];
+    if (C.word_bits == ch->word_bits && C.w == ch->w && C.s == ch->s && C.z == ch->z && C.k == ch->k)
+      return table_cost(*M, static_cast<int>(c), static_cast<double>(ch->m), static_cast<double>(n), tw_ns);
+  }
+  return -1.0;
+}
+
+// Persistence: a small binary file (magic, version, grid, tables, MLP weights).
+int dtl_model_save(const dtl_cost_model* M, const char* path) {
+  if (!M || !path) return DTL_EINVAL;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return DTL_ERUNTIME;
+  const uint32_t hdr[5] = {0x4C544448u /* "HDTL" */, 1u, static_cast<uint32_t>(M->cfgs.size()), M->log2_min,
+                           M->log2_max};
+  bool ok = std::fwrite(hdr, sizeof(hdr), 1, f) == 1;
+  ok = ok && std::fwrite(M->cfgs.data(), sizeof(Cfg), M->cfgs.size(), f) == M->cfgs.size();
+  ok = ok && std::fwrite(M->t_ns.data(), sizeof(double), M->t_ns.size(), f) == M->t_ns.size();
+  ok = ok && std::fwrite(M->fpr.data(), sizeof(double), M->fpr.size(), f) == M->fpr.size();
+  ok = ok && std::fwrite(M->train_cell.data(), 1, M->train_cell.size(), f) == M->train_cell.size();
+  const double norm[3] = {M->y_mean, M->y_std, M->calib_seconds};
+  ok = ok && std::fwrite(norm, sizeof(norm), 1, f) == 1;
+  const uint32_t has_mlp = M->mlp ? 1u : 0u;
+  ok = ok && std::fwrite(&has_mlp, 4, 1, f) == 1;
+  if (ok && has_mlp) {
+    std::vector<float> w(dtl_mlp_param_count());
+    ok = dtl_mlp_get_params(M->mlp, w.data()) == DTL_OK && std::fwrite(w.data(), 4, w.size(), f) == w.size();
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? DTL_OK : DTL_ERUNTIME;
+}
+
+int dtl_model_load(const char* path, dtl_cost_model** out) {
+  if (!path || !out) return DTL_EINVAL;
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return DTL_ERUNTIME;
+  uint32_t hdr[5];
+  dtl_cost_model* M = new dtl_cost_model();
+  bool ok = std::fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == 0x4C544448u && hdr[1] == 1u && hdr[2] < 100000 &&
+            hdr[3] <= hdr[4] && hdr[4] <= 40;
+  if (ok) {
+    M->cfgs.resize(hdr[2]);
+    M->log2_min = hdr[3];
+    M->log2_max = hdr[4];
+    const size_t ns = M->nsizes();
+    M->t_ns.resize(hdr[2] * ns);
+    M->fpr.resize(static_cast<size_t>(hdr[2]) * kNBpk);
+    M->train_cell.resize(hdr[2] * ns);
+    ok = std::fread(M->cfgs.data(), sizeof(Cfg), M->cfgs.size(), f) == M->cfgs.size() &&
+         std::fread(M->t_ns.data(), sizeof(double), M->t_ns.size(), f) == M->t_ns.size() &&
+         std::fread(M->fpr.data(), sizeof(double), M->fpr.size(), f) == M->fpr.size() &&
+         std::fread(M->train_cell.data(), 1, M->train_cell.size(), f) == M->train_cell.size();
+  }
+  double norm[3];
+  uint32_t has_mlp = 0;
+  ok = ok && std::fread(norm, sizeof(norm), 1, f) == 1 && std::fread(&has_mlp, 4, 1, f) == 1;
+  if (ok) {
+    M->y_mean = norm[0];
+    M->y_std = norm[1];
+    M->calib_seconds = norm[2];
+  }
+  if (ok && has_mlp) {
+    std::vector<float> w(dtl_mlp_param_count());
+    ok = std::fread(w.data(), 4, w.size(), f) == w.size() && dtl_mlp_create(0, &M->mlp) == DTL_OK &&
+         dtl_mlp_set_params(M->mlp, w.data()) == DTL_OK;
+  }
+  std::fclose(f);
+  if (!ok) {
+    dtl_model_destroy(M);
+    return DTL_ERUNTIME;
+  }
+  *out = M;
+  return DTL_OK;
+}
+
+}  // extern "C"

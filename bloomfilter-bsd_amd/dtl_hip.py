@@ -46,6 +46,24 @@ class BbfTask(C.Structure):
                 ("d_pos", C.c_void_p), ("d_count", C.c_void_p)]
 
 
+class CalibParams(C.Structure):
+    _fields_ = [("log2_m_min", C.c_uint32), ("log2_m_max", C.c_uint32), ("probe_keys_log2", C.c_uint32),
+                ("reps", C.c_uint32), ("fpr_keys_log2", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class BbfChoice(C.Structure):
+    _fields_ = [("word_bits", C.c_uint32), ("w", C.c_uint32), ("s", C.c_uint32), ("z", C.c_uint32),
+                ("k", C.c_uint32), ("reserved", C.c_uint32), ("m", C.c_uint64), ("cost_ns", C.c_double),
+                ("predicted", C.c_double)]
+
+    def key(self):
+        return (self.word_bits, self.w, self.s, self.z, self.k)
+
+    def as_dict(self):
+        return {"word_bits": self.word_bits, "w": self.w, "s": self.s, "z": self.z, "k": self.k,
+                "m": self.m, "cost_ns": round(self.cost_ns, 5)}
+
+
 class DtlError(RuntimeError):
     def __init__(self, fn, status):
         detail = load().dtl_last_error().decode() if status == EHIP else ""
@@ -58,6 +76,11 @@ EXPORTS = [
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
     "dtl_bbf_config_count", "dtl_version", "dtl_gather_bench",
+    "bbf_batch_plan_group_count", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
+    "dtl_model_fpr", "dtl_model_config_count", "dtl_model_get_config", "dtl_model_calibration_seconds",
+    "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
+    "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
+    "dtl_mlp_set_params",
 ]
 
 _lib = None
@@ -96,6 +119,40 @@ def load(path=LIB_PATH):
     L.dtl_bbf_config_count.argtypes = []
     L.dtl_version.argtypes = []
     L.dtl_gather_bench.argtypes = [vp, u64, u32, u64, vp, vp]
+    L.bbf_batch_plan_group_count.argtypes = [vp]
+    L.bbf_batch_plan_group_count.restype = u32
+    f64, f32 = C.c_double, C.c_float
+    L.dtl_model_calibrate.argtypes = [C.POINTER(CalibParams), C.POINTER(vp), vp]
+    L.dtl_model_train.argtypes = [vp, f64, u32, u32, u32, vp]
+    L.dtl_model_choose.argtypes = [vp, u64, f64, i32, C.POINTER(BbfChoice), vp]
+    L.dtl_model_cost.argtypes = [vp, C.POINTER(BbfChoice), u64, f64]
+    L.dtl_model_cost.restype = f64
+    L.dtl_model_lookup_ns.argtypes = [vp, i32, f64]
+    L.dtl_model_lookup_ns.restype = f64
+    L.dtl_model_fpr.argtypes = [vp, i32, f64]
+    L.dtl_model_fpr.restype = f64
+    L.dtl_model_config_count.argtypes = [vp]
+    L.dtl_model_get_config.argtypes = [vp, i32, C.POINTER(BbfChoice)]
+    L.dtl_model_calibration_seconds.argtypes = [vp]
+    L.dtl_model_calibration_seconds.restype = f64
+    L.dtl_model_save.argtypes = [vp, C.c_char_p]
+    L.dtl_model_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.dtl_model_destroy.argtypes = [vp]
+    L.dtl_model_destroy.restype = None
+    L.dtl_mlp_param_count.argtypes = []
+    L.dtl_mlp_create.argtypes = [u32, C.POINTER(vp)]
+    L.dtl_mlp_destroy.argtypes = [vp]
+    L.dtl_mlp_destroy.restype = None
+    L.dtl_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, f32, u32, vp]
+    L.dtl_mlp_predict.argtypes = [vp, vp, u32, vp, vp]
+    L.dtl_mlp_gradient.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.dtl_mlp_get_params.argtypes = [vp, C.POINTER(C.c_float)]
+    L.dtl_mlp_set_params.argtypes = [vp, C.POINTER(C.c_float)]
+    for name in ("dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_config_count",
+                 "dtl_model_get_config", "dtl_model_save", "dtl_model_load", "dtl_mlp_param_count",
+                 "dtl_mlp_create", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient",
+                 "dtl_mlp_get_params", "dtl_mlp_set_params"):
+        getattr(L, name).restype = i32
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
                  "cf_probe", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
                  "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_version",
@@ -240,6 +297,9 @@ class BatchPlan:
     def results(self):
         return [positions(o, c) for o, c in zip(self.outs, self.counts)]
 
+    def group_count(self):
+        return self.L.bbf_batch_plan_group_count(self.plan)
+
     def close(self):
         if self.plan:
             self.L.bbf_batch_plan_destroy(self.plan)
@@ -255,3 +315,119 @@ class BatchPlan:
 def gather_bench(buf, mode, n_reads, sink, stream=None):
     _check("dtl_gather_bench", load().dtl_gather_bench(_ptr(buf), buf.numel() * buf.element_size(), mode,
                                                        n_reads, _ptr(sink), _stream(stream)))
+
+
+class MLP:
+    """The library's FP32 8-64-64-1 MLP (dtl_mlp.hip)."""
+
+    def __init__(self, seed=0):
+        self.L = load()
+        self.h = C.c_void_p()
+        _check("dtl_mlp_create", self.L.dtl_mlp_create(seed, C.byref(self.h)))
+        self.n_params = self.L.dtl_mlp_param_count()
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check("dtl_mlp_get_params", self.L.dtl_mlp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check("dtl_mlp_set_params", self.L.dtl_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def predict(self, x):
+        import torch
+        y = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        _check("dtl_mlp_predict", self.L.dtl_mlp_predict(self.h, _ptr(x), x.shape[0], _ptr(y), _stream(None)))
+        return y
+
+    def gradient(self, x, t):
+        import torch
+        g = torch.empty(self.n_params, dtype=torch.float32, device=x.device)
+        _check("dtl_mlp_gradient", self.L.dtl_mlp_gradient(self.h, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
+                                                           _stream(None)))
+        return g
+
+    def train(self, x, t, epochs, batch, lr, seed=0):
+        _check("dtl_mlp_train", self.L.dtl_mlp_train(self.h, _ptr(x), _ptr(t), x.shape[0], epochs, batch,
+                                                     lr, seed, _stream(None)))
+
+    def close(self):
+        if self.h:
+            self.L.dtl_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CostModel:
+    """Calibrated cost table + learned MLP for choosing filter configurations."""
+
+    def __init__(self, handle):
+        self.L = load()
+        self.h = handle
+
+    @classmethod
+    def calibrate(cls, params=None):
+        L = load()
+        h = C.c_void_p()
+        pp = C.byref(params) if params is not None else None
+        _check("dtl_model_calibrate", L.dtl_model_calibrate(pp, C.byref(h), _stream(None)))
+        return cls(h)
+
+    @classmethod
+    def load_file(cls, path):
+        L = load()
+        h = C.c_void_p()
+        _check("dtl_model_load", L.dtl_model_load(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def save(self, path):
+        _check("dtl_model_save", self.L.dtl_model_save(self.h, path.encode()))
+
+    def train(self, train_fraction=0.25, samples=200000, epochs=40, seed=1):
+        _check("dtl_model_train", self.L.dtl_model_train(self.h, train_fraction, samples, epochs, seed,
+                                                         _stream(None)))
+
+    def choose(self, n, tw_ns, use_mlp=True):
+        ch = BbfChoice()
+        _check("dtl_model_choose", self.L.dtl_model_choose(self.h, n, tw_ns, int(use_mlp), C.byref(ch),
+                                                           _stream(None)))
+        return ch
+
+    def cost(self, choice, n, tw_ns):
+        return self.L.dtl_model_cost(self.h, C.byref(choice), n, tw_ns)
+
+    def config_count(self):
+        return self.L.dtl_model_config_count(self.h)
+
+    def config(self, i):
+        ch = BbfChoice()
+        _check("dtl_model_get_config", self.L.dtl_model_get_config(self.h, i, C.byref(ch)))
+        return ch
+
+    def lookup_ns(self, i, m):
+        return self.L.dtl_model_lookup_ns(self.h, i, m)
+
+    def fpr(self, i, bpk):
+        return self.L.dtl_model_fpr(self.h, i, bpk)
+
+    def calibration_seconds(self):
+        return self.L.dtl_model_calibration_seconds(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.dtl_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -158,20 +158,25 @@ typedef struct dtl_bbf_task {
 } dtl_bbf_task;
 
 /* A batch plan owns the device-side task table and scratch for a fixed set
- * of tasks; running it enqueues four kernels (clear, persistent probe over
- * all tasks, per-task scan, expand) with no allocation or memset, so a run
- * can be captured into a hipGraph and replayed.  Task buffers are referenced,
- * not copied: their contents may change between runs. */
+ * of tasks; running it enqueues, per configuration group, four kernels
+ * (clear, persistent probe over the group's tasks, per-task scan, expand)
+ * with no allocation or memset, so a run can be captured into a hipGraph and
+ * replayed.  Tasks may mix configurations (word_bits, w, s, z, k): they are
+ * grouped in order of first appearance.  Task buffers are referenced, not
+ * copied: their contents may change between runs. */
 typedef struct dtl_batch_plan dtl_batch_plan;
 
 int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
                           const dtl_probe_params* params, dtl_batch_plan** plan);
 int bbf_batch_plan_run(dtl_batch_plan* plan, dtl_stream_t stream);
 void bbf_batch_plan_destroy(dtl_batch_plan* plan);
+/* Number of configuration groups (persistent launches per run). */
+uint32_t bbf_batch_plan_group_count(const dtl_batch_plan* plan);
 
-/* Autotunes the plan's launch shape (LDS staging threshold, workgroup size,
- * tiles per grab) by timing `reps` runs of each candidate on the plan's own
- * tasks; the fastest shape is kept for later runs and copied to *best. */
+/* Autotunes each group's launch shape (LDS staging threshold, workgroup
+ * size, tiles per grab) by timing `reps` runs of each candidate on the
+ * group's own tasks; the fastest shapes are kept for later runs and the one
+ * of the group with the most keys is copied to *best. */
 int bbf_batch_plan_tune(dtl_batch_plan* plan, dtl_stream_t stream, uint32_t reps,
                         dtl_probe_params* best);
 
@@ -179,6 +184,58 @@ int bbf_batch_plan_tune(dtl_batch_plan* plan, dtl_stream_t stream, uint32_t reps
  * bbf_probe would produce for it. */
 int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_probe_params* params,
                       dtl_stream_t stream);
+
+/* ---- learned configuration selection ------------------------------------------ */
+/* No single reference counterpart: replaces the calibration sweep + skyline
+ * lookup that picks (word size, w, s, z, k, m) from (n, tw)
+ * (model/calibration.cpp:251-311, skyline_matrix.hpp:118-155,
+ * optimizer.hpp:25-140) with GPU-timed calibration, measured FPRs and an FP32
+ * 8-64-64-1 MLP; the calibrated table is the fallback and the judge. */
+typedef struct dtl_calib_params {
+  uint32_t log2_m_min;      /* smallest calibrated filter, log2 bits (default 13) */
+  uint32_t log2_m_max;      /* largest, log2 bits (default 33 = 1 GiB) */
+  uint32_t probe_keys_log2; /* keys per timing run (default 24) */
+  uint32_t reps;            /* timed repetitions, median kept (default 3) */
+  uint32_t fpr_keys_log2;   /* absent keys per FPR measurement (default 22) */
+  uint32_t flags;
+} dtl_calib_params;
+
+typedef struct dtl_bbf_choice {
+  uint32_t word_bits, w, s, z, k, reserved;
+  uint64_t m;               /* filter length in bits */
+  double cost_ns;           /* calibrated-table overhead per probe: t_lookup + tw * fpr */
+  double predicted;         /* the chooser's own estimate (MLP or table) */
+} dtl_bbf_choice;
+
+typedef struct dtl_cost_model dtl_cost_model;
+
+int dtl_model_calibrate(const dtl_calib_params* params, dtl_cost_model** model, dtl_stream_t stream);
+int dtl_model_train(dtl_cost_model* model, double train_fraction, uint32_t samples, uint32_t epochs,
+                    uint32_t seed, dtl_stream_t stream);
+int dtl_model_choose(const dtl_cost_model* model, uint64_t n, double tw_ns, int use_mlp,
+                     dtl_bbf_choice* choice, dtl_stream_t stream);
+double dtl_model_cost(const dtl_cost_model* model, const dtl_bbf_choice* choice, uint64_t n, double tw_ns);
+double dtl_model_lookup_ns(const dtl_cost_model* model, int config, double m);
+double dtl_model_fpr(const dtl_cost_model* model, int config, double bits_per_key);
+int dtl_model_config_count(const dtl_cost_model* model);
+int dtl_model_get_config(const dtl_cost_model* model, int config, dtl_bbf_choice* out);
+double dtl_model_calibration_seconds(const dtl_cost_model* model);
+int dtl_model_save(const dtl_cost_model* model, const char* path);
+int dtl_model_load(const char* path, dtl_cost_model** model);
+void dtl_model_destroy(dtl_cost_model* model);
+
+/* The MLP itself (FP32, parameters W1[64][8] b1[64] W2[64][64] b2[64] W3[64] b3). */
+typedef struct dtl_mlp dtl_mlp;
+int dtl_mlp_param_count(void);
+int dtl_mlp_create(uint32_t seed, dtl_mlp** mlp);
+void dtl_mlp_destroy(dtl_mlp* mlp);
+int dtl_mlp_train(dtl_mlp* mlp, const float* d_x, const float* d_t, uint32_t n, uint32_t epochs,
+                  uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream);
+int dtl_mlp_predict(const dtl_mlp* mlp, const float* d_x, uint32_t n, float* d_y, dtl_stream_t stream);
+int dtl_mlp_gradient(const dtl_mlp* mlp, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
+                     dtl_stream_t stream);
+int dtl_mlp_get_params(const dtl_mlp* mlp, float* h_params);
+int dtl_mlp_set_params(dtl_mlp* mlp, const float* h_params);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* Random-sector read microbenchmark (no reference counterpart; used to

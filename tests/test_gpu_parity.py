@@ -305,18 +305,23 @@ def test_full_size_1gib_properties():
     isin = torch.zeros(n_probe, dtype=torch.bool, device="cuda")
     isin[pos.long()] = True
     assert bool(isin[idx].all())
+    # At this size (2^24 blocks chosen by 24 bits of a bijective 32-bit
+    # multiplicative hash) only 8 hash bits of entropy are left inside a
+    # block, and the reference's filter has a false-positive rate of ~0.03%,
+    # far below the 0.42% of the random-hash formula (fpr.hpp:11-60); the
+    # oracle measures the same on the CPU.  So the rate is bounded here and the
+    # exact check is the prefix comparison below.
     fpr = (c - hits) / (n_probe - hits)
     model = O.fpr_config(desc.m_actual, n_build, 8, 8, 8, 8, 1)
-    assert abs(fpr - model) / model < 0.05, (fpr, model)
-    # exact agreement with the oracle on a sample of keys
+    assert 0.01 * model < fpr < model, (fpr, model)
+    # exact agreement with the oracle's batch_contains on the first 2^21 keys
     host_words = words.cpu().numpy().view(np.uint64)
-    sample = torch.randint(0, n_probe, (200000,), device="cuda", generator=g)
-    sk = probe[sample].cpu().numpy().view(np.uint32)
-    si = isin[sample].cpu().numpy()
-    for key, flag in zip(sk[:50000].tolist(), si[:50000].tolist()):
-        assert f.contains(host_words, key) == flag
+    pre = 1 << 21
+    exp = f.batch_contains(host_words, probe[:pre].cpu().numpy().view(np.uint32), 0)
+    got = pos[pos < pre].cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, exp)
     bs = build[torch.randint(0, n_build, (20000,), device="cuda", generator=g)].cpu().numpy().view(np.uint32)
-    assert all(f.contains(host_words, k) for k in bs.tolist())
+    assert len(f.batch_contains(host_words, bs, 0)) == len(bs)
 
 
 def test_batched_autotune_keeps_results():
@@ -338,4 +343,47 @@ def test_batched_autotune_keeps_results():
     torch.cuda.synchronize()
     for g, e in zip(plan.results(), expect):
         assert np.array_equal(g, e)
+    plan.close()
+
+
+MIXED_CFGS = [(32, 6, 16, 16, 2), (64, 8, 8, 8, 1), (32, 4, 1, 1, 1), (64, 12, 8, 8, 4), (32, 3, 2, 1, 1),
+              (64, 10, 4, 2, 1)]
+
+
+def test_batched_mixed_configurations():
+    """One plan over tasks of several configurations (interleaved), tuned and
+    graph-replayed: every list equals the oracle's."""
+    rng = np.random.default_rng(99)
+    tasks, expect = [], []
+    for i in range(30):
+        wb, k, w, s, z = MIXED_CFGS[i % len(MIXED_CFGS)]
+        n = int(2 ** rng.uniform(7, 15))
+        m = int(rng.integers(6, 16)) * n
+        f = O.Bloom(wb, m, k, w, s, z)
+        desc = D.bbf_desc(wb, m, k, w, s, z)
+        build = rng.integers(0, 2**31, n, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        probe = mixed_keys(build, int(2 ** rng.uniform(8, 15)), rng.uniform(0.01, 0.3), i)
+        tasks.append((desc, words, to_dev(probe), 7 * i))
+        expect.append(f.batch_contains(oracle_words(f, build), probe, 7 * i))
+    plan = D.BatchPlan(tasks)
+    assert plan.group_count() == len(MIXED_CFGS)
+    plan.tune(reps=1)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.run(s)
+    torch.cuda.synchronize()
+    for g, e in zip(plan.results(), expect):
+        assert np.array_equal(g, e)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        plan.run(s)
+    for rep in range(2):
+        for c in plan.counts:
+            c.fill_(-1)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for g, e in zip(plan.results(), expect):
+            assert np.array_equal(g, e)
     plan.close()
