@@ -79,6 +79,8 @@ def time_probe(torch, D, desc, words, probe, steps, warmup, params=None):
     for i in range(steps):
         p = D.ProbeParams() if params is None else D.ProbeParams(
             params.lds_max_bytes, params.block_threads, params.grid_blocks, params.chunk_tiles, params.flags)
+        if params is not None:
+            p.slice_log2 = params.slice_log2
         p.ev_start = evs[i][0].cuda_event
         p.ev_stop = evs[i][1].cuda_event
         D.bbf_probe(desc, words, probe, 0, params=p, out=out, count=cnt)

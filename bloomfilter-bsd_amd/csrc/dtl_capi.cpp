@@ -374,6 +374,82 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
   const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
   const size_t cnt_bytes = static_cast<size_t>(ntiles) * 4;
+  const uint32_t flags = params ? params->flags : 0u;
+  uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
+  if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
+  const bool use_lds = !(flags & DTL_PROBE_FORCE_GLOBAL) && d->size_bytes <= lds_max;
+
+  // partitioned probe: slices of 2^slice_log2 bytes (at most kPartMax of them)
+  uint32_t slice_shift = 0, nparts = 0;
+  if (!use_lds && !(flags & DTL_PROBE_DIRECT) && e->probe_part) {
+    const bool want = (flags & DTL_PROBE_PARTITIONED) ||
+                      (d->size_bytes >= (64ull << 20) && n >= (1ull << 22));
+    // slices of 2 MiB, 4 MiB from 512 MiB on (measured on MI355X)
+    const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
+                                : d->size_bytes >= (512ull << 20) ? 22u : 21u;
+    const uint32_t bb_log2 = floor_log2(block_bytes(d));
+    slice_shift = slice_log2 > bb_log2 ? slice_log2 - bb_log2 : 0u;
+    while (slice_shift < 31 && ((static_cast<uint64_t>(d->block_cnt) + (1ull << slice_shift) - 1) >> slice_shift) > kPartMax)
+      ++slice_shift;
+    nparts = static_cast<uint32_t>((static_cast<uint64_t>(d->block_cnt) + (1ull << slice_shift) - 1) >> slice_shift);
+    if (!want || nparts < 2) nparts = 0;
+  }
+  if (nparts) {
+    const uint32_t nbuckets = (ntiles + kBucketTiles - 1) / kBucketTiles;
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t cnt_r = up(cnt_bytes);
+    const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
+    const size_t part_bytes = up((nbins * 3 + 1) * 4);
+    const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
+    const size_t pair_bytes = up(static_cast<size_t>(n) * 4);
+    void* scratch = nullptr;
+    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + 2 * pair_bytes, &scratch);
+    if (rc != DTL_OK) return rc;
+    char* sp = static_cast<char*>(scratch);
+    BbfPartLaunch P;
+    P.words = d_words;
+    P.keys = d_keys;
+    P.n = static_cast<uint32_t>(n);
+    fill_addr(d, &P.addr_magic, &P.addr_mshift, &P.addr_divisor, &P.addr_post);
+    P.slice_shift = slice_shift;
+    P.nparts = nparts;
+    P.scatter_threads = (params && params->block_threads) ? params->block_threads : 1024u;
+    P.bits = reinterpret_cast<uint64_t*>(sp);
+    sp += bits_bytes;
+    P.tile_cnt = reinterpret_cast<uint32_t*>(sp);
+    sp += cnt_r;
+    P.pbits = reinterpret_cast<uint64_t*>(sp);
+    sp += bits_bytes;
+    P.ptile_cnt = reinterpret_cast<uint32_t*>(sp);
+    sp += cnt_r;
+    P.counts = reinterpret_cast<uint32_t*>(sp);
+    P.offsets = P.counts + nbins;
+    P.cursor = P.offsets + nbins + 1;
+    sp += part_bytes;
+    P.d_hits = reinterpret_cast<uint64_t*>(sp);
+    P.nbuckets = nbuckets;
+    P.bcounts = reinterpret_cast<uint32_t*>(sp + 8);
+    P.boffsets = P.bcounts + nbuckets;
+    P.bcursor = P.boffsets + nbuckets + 1;
+    sp += bucket_bytes;
+    P.keys_p = reinterpret_cast<uint32_t*>(sp);
+    P.pos_p = reinterpret_cast<uint32_t*>(sp + pair_bytes);
+    P.ntiles = ntiles;
+    P.cu_count = di.cu_count;
+    P.stream = st;
+    if (params && params->ev_start &&
+        hipEventRecord(static_cast<hipEvent_t>(params->ev_start), st) != hipSuccess)
+      return DTL_EHIP;
+    hipError_t he = partition_keys(P);
+    if (he == hipSuccess) he = e->probe_part(P);
+    if (he == hipSuccess) he = part_reorder_matches(P);
+    if (he != hipSuccess) return hip_status(he);
+    if (params && params->ev_stop &&
+        hipEventRecord(static_cast<hipEvent_t>(params->ev_stop), st) != hipSuccess)
+      return DTL_EHIP;
+    return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));
+  }
+
   void* scratch = nullptr;
   rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
   if (rc != DTL_OK) return rc;
@@ -386,10 +462,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   L.tile_cnt = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + bits_bytes);
   L.ntiles = ntiles;
   L.filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d->size_bytes, 0xFFFFFFFFull));
-  uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
-  if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
-  const bool force_global = params && (params->flags & DTL_PROBE_FORCE_GLOBAL);
-  L.use_lds = !force_global && d->size_bytes <= lds_max;
+  L.use_lds = use_lds;
   uint32_t threads = (params && params->block_threads) ? params->block_threads : 0;
   if (threads == 0) threads = d->size_bytes > 40 * 1024 ? 1024 : 256;
   L.lds_threads = threads >= 1024 ? 1024 : 256;

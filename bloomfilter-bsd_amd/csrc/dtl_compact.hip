@@ -11,37 +11,9 @@
 
 #include "dtl_internal.hpp"
 #include "dtl_kernels.hpp"
+#include "dtl_scan.hpp"
 
 namespace dtlh {
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = threadIdx.x & 63u;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= static_cast<uint32_t>(d)) v += t;
-  }
-  return v;
-}
-
-// Exclusive scan over the workgroup in thread order; `total` receives the sum.
-template <int BT>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* sh) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t incl = wave_incl_scan(v);
-  if (lane == 63) sh[wave] = incl;
-  __syncthreads();
-  if (wave == 0) {
-    uint32_t x = lane < static_cast<uint32_t>(BT / 64) ? sh[lane] : 0u;
-    x = wave_incl_scan(x);
-    if (lane < static_cast<uint32_t>(BT / 64)) sh[lane] = x;
-  }
-  __syncthreads();
-  const uint32_t before = wave ? sh[wave - 1] : 0u;
-  total = sh[BT / 64 - 1];
-  __syncthreads();
-  return before + incl - v;
-}
 
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
@@ -103,6 +75,31 @@ __device__ __forceinline__ void expand_tile(const uint64_t* __restrict__ bits, u
   }
 }
 
+// As expand_tile, but writes src[i] (a gathered value) instead of position i.
+__device__ __forceinline__ void expand_tile_gather(const uint64_t* __restrict__ bits, uint32_t tile,
+                                                   uint32_t offset, const uint32_t* __restrict__ src,
+                                                   uint32_t* __restrict__ out, uint32_t* sh) {
+  const uint32_t t = threadIdx.x;
+  const uint64_t word = bits[static_cast<uint64_t>(tile) * kTileWords + (t >> 3)];
+  uint32_t b8 = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
+  uint32_t tot;
+  uint32_t o = offset + block_excl_scan<kExpandThreads>(__popc(b8), tot, sh);
+  const uint64_t kb = static_cast<uint64_t>(tile) * kTileKeys + (t >> 3) * 64u + (t & 7u) * 8u;
+  while (b8) {
+    const uint32_t j = __ffs(b8) - 1u;
+    out[o++] = src[kb + j];
+    b8 &= b8 - 1u;
+  }
+}
+
+__global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t* __restrict__ bits,
+                                                                  const uint32_t* __restrict__ offs,
+                                                                  const uint32_t* __restrict__ src,
+                                                                  uint32_t* __restrict__ out) {
+  __shared__ uint32_t sh[kExpandThreads / 64];
+  expand_tile_gather(bits, blockIdx.x, offs[blockIdx.x], src, out, sh);
+}
+
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ offs,
                                                                   uint32_t match_offset,
@@ -135,6 +132,17 @@ hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t nti
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_expand_single, dim3(ntiles), dim3(kExpandThreads), 0, stream, bits,
                      tile_cnt, match_offset, pos);
+  return hipGetLastError();
+}
+
+hipError_t scan_tile_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_total, hipStream_t stream) {
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, stream, tile_cnt, ntiles, d_total);
+  return hipGetLastError();
+}
+
+hipError_t expand_gather(const uint64_t* bits, const uint32_t* offs, uint32_t ntiles, const uint32_t* src,
+                         uint32_t* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_expand_gather, dim3(ntiles), dim3(kExpandThreads), 0, stream, bits, offs, src, out);
   return hipGetLastError();
 }
 

@@ -69,12 +69,53 @@ struct BbfBatchLaunch {
   hipStream_t stream;
 };
 
+// Partitioned probe of one large filter (dtl_partition.hip).
+struct BbfPartLaunch {
+  const void* words;
+  const uint32_t* keys;
+  uint32_t n;
+  uint32_t addr_magic, addr_mshift, addr_divisor, addr_post;
+  uint32_t slice_shift;   // partition = block index >> slice_shift
+  uint32_t nparts;
+  uint32_t scatter_threads;  // 256, 512 or 1024 (tile = 16 keys per thread)
+  uint32_t* counts;       // nparts * kPartLanes bins
+  uint32_t* offsets;      // bins + 1
+  uint32_t* cursor;       // bins
+  uint32_t* keys_p;       // n keys grouped by partition (later: hit positions)
+  uint32_t* pos_p;        // their original positions (later: hit positions by bucket)
+  uint64_t* pbits;        // match bitmap in partitioned order, ntiles * kTileWords words
+  uint32_t* ptile_cnt;    // ntiles
+  uint64_t* d_hits;       // device: number of matches
+  uint32_t nbuckets;      // position buckets of kBucketKeys
+  uint32_t* bcounts;      // nbuckets
+  uint32_t* boffsets;     // nbuckets + 1
+  uint32_t* bcursor;      // nbuckets
+  uint64_t* bits;         // match bitmap in key order, ntiles * kTileWords words
+  uint32_t* tile_cnt;     // ntiles
+  uint32_t ntiles;
+  int cu_count;
+  hipStream_t stream;
+};
+
+constexpr uint32_t kBucketShift = 16;                 // positions per bucket of the hit sort
+constexpr uint32_t kBucketTiles = (1u << kBucketShift) / kTileKeys;
+
 struct BbfEntry {
   uint32_t word_bits, w, s, z, k;
   hipError_t (*probe)(const BbfProbeLaunch&);
   hipError_t (*build)(const BbfBuildLaunch&);
   hipError_t (*probe_batched)(const BbfBatchLaunch&);  // may be null
+  hipError_t (*probe_part)(const BbfPartLaunch&);      // probe phase of the partitioned path
 };
+
+constexpr uint32_t kPartMax = 2048;   // partitions of the partitioned probe
+constexpr uint32_t kPartLanes = 8;    // sub-ranges per partition (one per XCD)
+
+// Partitioned probe phases that do not depend on the configuration:
+// histogram + scan + scatter of the keys, and the reordering of the matches
+// (partitioned-order bitmap -> positions -> key-order bitmap + tile counts).
+hipError_t partition_keys(const BbfPartLaunch& L);
+hipError_t part_reorder_matches(const BbfPartLaunch& L);
 
 struct CfProbeLaunch {
   const void* buf;
@@ -127,6 +168,12 @@ hipError_t compact_tasks(const uint64_t* bits, uint32_t* tile_cnt, const BatchTa
                          const uint32_t* tile_task, uint32_t n_tasks, uint32_t total_tiles,
                          hipStream_t stream);
 hipError_t write_zero_count(uint64_t* d_count, hipStream_t stream);
+// Exclusive scan of tile counts in place, total to *d_total.
+hipError_t scan_tile_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_total, hipStream_t stream);
+// For every set bit i of the bitmap (tile offsets from scan_tile_counts):
+// out[rank] = src[i], in bit order.
+hipError_t expand_gather(const uint64_t* bits, const uint32_t* offs, uint32_t ntiles, const uint32_t* src,
+                         uint32_t* out, hipStream_t stream);
 hipError_t zero_u32(uint32_t* p, hipStream_t stream);
 
 // Raises a kernel's dynamic-LDS limit when `bytes` exceeds the default

@@ -163,7 +163,13 @@ __device__ __forceinline__ uint32_t block_sum_waves(uint32_t v, uint32_t* red) {
 
 // ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter read from global memory.
-// grid = number of tiles, block = 256 threads (8 rounds x 4 waves = 32 words).
+// block = 256 threads (8 rounds x 4 waves = 32 bitmap words per tile).
+// tiles_per_xcd == 0: workgroup b probes tile b (grid = ntiles).
+// tiles_per_xcd > 0 (partitioned probe, keys grouped by filter slice):
+// workgroup b runs on XCD b % 8 (round-robin dispatch) and probes tile
+// (b % 8) * tiles_per_xcd + b / 8, so each XCD walks a contiguous range of
+// the grouped keys and the filter slices of that range are cached in its
+// own L2 one after another (grid = 8 * tiles_per_xcd).
 // ---------------------------------------------------------------------------
 constexpr int kGlobalThreads = 256;
 constexpr int kGlobalUnroll = 4;
@@ -171,14 +177,18 @@ constexpr int kGlobalUnroll = 4;
 template <class C>
 __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
     const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
-    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt) {
+    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
+    uint32_t tiles_per_xcd, uint32_t ntiles) {
   using W = typename C::word_t;
   constexpr int BT = kGlobalThreads;
   constexpr int ROUNDS = kTileKeys / BT;
   constexpr int U = kGlobalUnroll;
   __shared__ uint32_t red[BT / 64];
+  const uint32_t tile =
+      tiles_per_xcd ? (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kTileKeys;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
   uint32_t cnt = 0;
 #pragma unroll
   for (int r0 = 0; r0 < ROUNDS; r0 += U) {
@@ -205,7 +215,7 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
     }
   }
   const uint32_t total = block_sum_waves<BT>(cnt, red);
-  if (tid == 0) tile_cnt[blockIdx.x] = total;
+  if (tid == 0) tile_cnt[tile] = total;
 }
 
 // ---------------------------------------------------------------------------

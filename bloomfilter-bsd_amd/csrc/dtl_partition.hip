@@ -1,0 +1,305 @@
+// dtl_partition.hip -- partitioned probing of blocked Bloom filters that do
+// not fit in LDS or L2.
+//
+// A probe of a large filter reads one random block per key; on gfx950 every
+// such read fills a whole 128-byte L2 line from HBM, so the direct kernel is
+// bound by random-line throughput.  The partitioned probe first groups the
+// keys by filter slice (slice = 2^slice_shift consecutive blocks, 2 MiB by
+// default), then probes slice after slice, so that a slice is read from HBM
+// into one XCD's L2 once and every key of the partition hits it there.
+// Only key streams touch HBM: keys are read twice (histogram, scatter), the
+// (key, position) pairs written and read once.
+//
+//   1. k_fill_zero_u32  clears the partition counts;
+//   2. k_part_hist      LDS histogram of partition ids, added to global counts;
+//   3. k_part_scan      exclusive scan -> partition offsets (and cursors);
+//   4. k_part_scatter   per tile of 16 keys per thread: counting sort in LDS, one global
+//                       atomicAdd per non-empty bin reserves the tile's run in
+//                       each partition, runs written out coalesced;
+//   5. k_bbf_probe_global (per configuration, dtl_kernels.hpp) probes the
+//                       grouped keys in XCD order and writes the match bitmap
+//                       in grouped order (ballot words, no atomics);
+//   6. reordering of the matches (only matches move, about 1-2% of keys):
+//      scan + k_expand_gather list the original positions of the matches,
+//      k_hit_hist / k_part_scan / k_hit_scatter bucket them by position
+//      (2^16 positions per bucket), and k_hit_bits builds each bucket's piece
+//      of the key-order bitmap in LDS and writes it with its tile counts.
+// The order of keys inside a partition depends on atomic timing, but the
+// key-order bitmap does not, so the match list is the same ascending list as
+// the direct probe's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dtl_internal.hpp"
+#include "dtl_kernels.hpp"
+#include "dtl_scan.hpp"
+
+namespace dtlh {
+
+constexpr int kHistThreads = 256;
+constexpr int kHistUnroll = 4;
+constexpr int kScatterKpt = 16;  // keys per thread of the scatter (tile = threads * 16 keys)
+constexpr int kPartScanThreads = 1024;
+
+__device__ __forceinline__ uint32_t part_of(uint32_t key, const AddrArgs& a, uint32_t slice_shift) {
+  return block_of(key * kMul.v[0], a) >> slice_shift;
+}
+
+__global__ void k_fill_zero_u32(uint32_t* __restrict__ p, uint64_t count) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    p[i] = 0u;
+}
+
+// Bins are (partition, lane) pairs, bin = partition * kLanes + lane, where
+// lane = (scatter tile index) % kLanes: each partition's range is split into
+// kLanes sub-ranges, one per XCD the scatter tiles of that lane run on
+// (round-robin dispatch), so the runs one XCD writes are adjacent and merge
+// into whole lines in its L2.  The split only affects speed: the hist and
+// scatter kernels derive the lane from the tile index the same way.
+constexpr uint32_t kLanes = kPartLanes;
+
+__device__ __forceinline__ uint32_t lane_of(uint64_t i, uint32_t tile_log2) {
+  return static_cast<uint32_t>(i >> tile_log2) & (kLanes - 1u);
+}
+
+// Each workgroup counts keys [blockIdx * per_wg, +per_wg) (per_wg is a
+// multiple of kHistThreads * kHistUnroll) into nparts * kLanes bins.
+__global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                            AddrArgs a, uint32_t slice_shift,
+                                                            uint32_t nparts, uint32_t per_wg,
+                                                            uint32_t tile_log2,
+                                                            uint32_t* __restrict__ counts) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nbins = nparts * kLanes;
+  for (uint32_t i = tid; i < nbins; i += kHistThreads) hist[i] = 0u;
+  __syncthreads();
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * per_wg;
+  const uint64_t end = std::min<uint64_t>(begin + per_wg, n);
+  for (uint64_t base = begin; base < end; base += kHistThreads * kHistUnroll) {
+    uint32_t key[kHistUnroll];
+    bool valid[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
+      valid[u] = i < end;
+      key[u] = load_key(keys, valid[u] ? i : begin);
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
+      if (valid[u]) atomicAdd(&hist[part_of(key[u], a, slice_shift) * kLanes + lane_of(i, tile_log2)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nbins; i += kHistThreads)
+    if (hist[i]) atomicAdd(&counts[i], hist[i]);
+}
+
+// One workgroup: offsets[p] = sum of counts[< p], offsets[nparts] = total;
+// cursor[p] = offsets[p].
+__global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* __restrict__ counts,
+                                                                uint32_t nparts,
+                                                                uint32_t* __restrict__ offsets,
+                                                                uint32_t* __restrict__ cursor) {
+  __shared__ uint32_t sh[kPartScanThreads / 64];
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nparts; c0 += kPartScanThreads) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < nparts ? counts[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = carry + block_excl_scan<kPartScanThreads>(v, tot, sh);
+    if (i < nparts) {
+      offsets[i] = ex;
+      cursor[i] = ex;
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) offsets[nparts] = carry;
+}
+
+// LDS layout (dynamic): hist | lbase | gbase (nparts u32 each) | skey (tile
+// u32) | sidx (tile u16).
+template <int kScatterThreads>
+__global__ __launch_bounds__(kScatterThreads) void k_part_scatter(const uint32_t* __restrict__ keys,
+                                                                  uint32_t n, AddrArgs a,
+                                                                  uint32_t slice_shift, uint32_t nparts,
+                                                                  uint32_t* __restrict__ cursor,
+                                                                  uint32_t* __restrict__ keys_p,
+                                                                  uint32_t* __restrict__ pos_p) {
+  constexpr uint32_t kScatterTile = kScatterThreads * kScatterKpt;
+  extern __shared__ uint32_t sm[];
+  __shared__ uint32_t red[kScatterThreads / 64];
+  uint32_t* hist = sm;
+  uint32_t* lbase = sm + nparts;
+  uint32_t* gbase = sm + 2 * nparts;
+  uint32_t* skey = sm + 3 * nparts;
+  uint16_t* sidx = reinterpret_cast<uint16_t*>(skey + kScatterTile);
+  const uint32_t tid = threadIdx.x;
+  const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kScatterTile;
+  for (uint32_t i = tid; i < nparts; i += kScatterThreads) hist[i] = 0u;
+  __syncthreads();
+  uint32_t key[kScatterKpt], pid[kScatterKpt], rank[kScatterKpt];
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
+    key[r] = load_key(keys, i < n ? i : n - 1);
+  }
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
+    pid[r] = part_of(key[r], a, slice_shift);
+    rank[r] = i < n ? atomicAdd(&hist[pid[r]], 1u) : 0u;
+  }
+  __syncthreads();
+  // exclusive scan of the histogram (each thread owns `per` consecutive bins)
+  // and reservation of this tile's run in every non-empty partition
+  const uint32_t per = (nparts + kScatterThreads - 1) / kScatterThreads;
+  const uint32_t b0 = tid * per, b1 = std::min(nparts, b0 + per);
+  uint32_t local = 0;
+  for (uint32_t b = b0; b < b1; ++b) local += hist[b];
+  uint32_t tot;
+  uint32_t run = block_excl_scan<kScatterThreads>(local, tot, red);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t c = hist[b];
+    lbase[b] = run;
+    if (c) gbase[b] = atomicAdd(&cursor[b * kLanes + (blockIdx.x & (kLanes - 1u))], c);
+    run += c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
+    if (i < n) {
+      const uint32_t slot = lbase[pid[r]] + rank[r];
+      skey[slot] = key[r];
+      sidx[slot] = static_cast<uint16_t>(r * kScatterThreads + tid);
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
+  for (uint32_t slot = tid; slot < cnt; slot += kScatterThreads) {
+    const uint32_t k = skey[slot];
+    const uint32_t p = part_of(k, a, slice_shift);
+    const uint32_t dst = gbase[p] + (slot - lbase[p]);
+    __builtin_nontemporal_store(k, keys_p + dst);
+    __builtin_nontemporal_store(static_cast<uint32_t>(tile_base) + sidx[slot], pos_p + dst);
+  }
+}
+
+// Matches (original positions, hp[0 .. *d_hits)) -> bucket histogram.
+__global__ __launch_bounds__(256) void k_hit_hist(const uint32_t* __restrict__ hp, const uint64_t* __restrict__ d_hits,
+                                                  uint32_t* __restrict__ bcounts) {
+  const uint64_t h = *d_hits;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h; i += stride)
+    atomicAdd(&bcounts[hp[i] >> kBucketShift], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_hit_scatter(const uint32_t* __restrict__ hp,
+                                                     const uint64_t* __restrict__ d_hits,
+                                                     uint32_t* __restrict__ bcursor, uint32_t* __restrict__ out) {
+  const uint64_t h = *d_hits;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h; i += stride) {
+    const uint32_t p = hp[i];
+    out[atomicAdd(&bcursor[p >> kBucketShift], 1u)] = p;
+  }
+}
+
+// One workgroup per bucket of 2^kBucketShift positions: its matches set bits
+// of an LDS bitmap, which is written out whole (zeros included) together
+// with the match count of each of its kBucketTiles tiles.
+constexpr uint32_t kBucketWords32 = (1u << kBucketShift) / 32u;
+
+__global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ sorted,
+                                                  const uint32_t* __restrict__ boffsets, uint32_t ntiles,
+                                                  uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t lbits[kBucketWords32];
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  for (uint32_t i = tid; i < kBucketWords32; i += 256) lbits[i] = 0u;
+  __syncthreads();
+  const uint32_t beg = boffsets[b], end = boffsets[b + 1];
+  for (uint32_t i = beg + tid; i < end; i += 256) {
+    const uint32_t p = sorted[i] & ((1u << kBucketShift) - 1u);
+    atomicOr(&lbits[p >> 5], 1u << (p & 31u));
+  }
+  __syncthreads();
+  const uint32_t tile0 = b * kBucketTiles;
+  const uint32_t tiles = std::min(kBucketTiles, ntiles - tile0);
+  for (uint32_t w = tid; w < tiles * kTileWords; w += 256) {
+    const uint64_t v = static_cast<uint64_t>(lbits[2 * w]) | (static_cast<uint64_t>(lbits[2 * w + 1]) << 32);
+    bits[static_cast<uint64_t>(tile0) * kTileWords + w] = v;
+  }
+  if (tid < tiles) {
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < 2 * kTileWords; ++j) c += __popc(lbits[tid * 2 * kTileWords + j]);
+    tile_cnt[tile0 + tid] = c;
+  }
+}
+
+static uint32_t fill_grid(uint64_t count) {
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((count + 255) / 256, 8192)));
+}
+
+static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 6u; }
+
+template <int BT>
+static hipError_t launch_scatter(const BbfPartLaunch& L, const AddrArgs& a) {
+  constexpr uint32_t tile = BT * kScatterKpt;
+  const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile);
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT>), lds);
+  if (e != hipSuccess) return e;
+  const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
+  hipLaunchKernelGGL(k_part_scatter<BT>, dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, a, L.slice_shift,
+                     L.nparts, L.cursor, L.keys_p, L.pos_p);
+  return hipGetLastError();
+}
+
+hipError_t partition_keys(const BbfPartLaunch& L) {
+  const AddrArgs a{L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post};
+  const hipStream_t st = L.stream;
+  const uint32_t nbins = L.nparts * kLanes;
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(nbins)), dim3(256), 0, st, L.counts,
+                     static_cast<uint64_t>(nbins));
+  // histogram: about 8 workgroups per CU, each a multiple of 1024 keys
+  const uint64_t unit = kHistThreads * kHistUnroll;
+  const uint64_t want_wg = static_cast<uint64_t>(std::max(L.cu_count, 1)) * 8;
+  uint64_t per_wg = (L.n + want_wg - 1) / want_wg;
+  per_wg = std::max<uint64_t>(unit, (per_wg + unit - 1) / unit * unit);
+  const uint32_t hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
+  const uint32_t threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
+  const uint32_t tile_log2 = threads == 1024 ? 14u : threads == 512 ? 13u : 12u;
+  hipLaunchKernelGGL(k_part_hist, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, a,
+                     L.slice_shift, L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.counts, nbins, L.offsets,
+                     L.cursor);
+  switch (threads) {
+    case 1024: return launch_scatter<1024>(L, a);
+    case 512: return launch_scatter<512>(L, a);
+    default: return launch_scatter<256>(L, a);
+  }
+}
+
+hipError_t part_reorder_matches(const BbfPartLaunch& L) {
+  const hipStream_t st = L.stream;
+  hipError_t e = scan_tile_counts(L.ptile_cnt, L.ntiles, L.d_hits, st);
+  // hp (the matches' original positions) overwrites the grouped keys
+  if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ntiles, L.pos_p, L.keys_p, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(L.nbuckets)), dim3(256), 0, st, L.bcounts,
+                     static_cast<uint64_t>(L.nbuckets));
+  const uint32_t grid = static_cast<uint32_t>(std::max(L.cu_count, 1)) * 4u;
+  hipLaunchKernelGGL(k_hit_hist, dim3(grid), dim3(256), 0, st, L.keys_p, L.d_hits, L.bcounts);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, L.nbuckets, L.boffsets,
+                     L.bcursor);
+  // bucketed positions overwrite the grouped positions
+  hipLaunchKernelGGL(k_hit_scatter, dim3(grid), dim3(256), 0, st, L.keys_p, L.d_hits, L.bcursor, L.pos_p);
+  hipLaunchKernelGGL(k_hit_bits, dim3(L.nbuckets), dim3(256), 0, st, L.pos_p, L.boffsets, L.ntiles, L.bits,
+                     L.tile_cnt);
+  return hipGetLastError();
+}
+
+}  // namespace dtlh

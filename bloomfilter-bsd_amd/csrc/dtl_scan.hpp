@@ -1,0 +1,39 @@
+// dtl_scan.hpp -- wavefront and workgroup prefix sums (64-wide waves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dtlh {
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= static_cast<uint32_t>(d)) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over the workgroup in thread order; `total` receives the sum.
+template <int BT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* sh) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) sh[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t x = lane < static_cast<uint32_t>(BT / 64) ? sh[lane] : 0u;
+    x = wave_incl_scan(x);
+    if (lane < static_cast<uint32_t>(BT / 64)) sh[lane] = x;
+  }
+  __syncthreads();
+  const uint32_t before = wave ? sh[wave - 1] : 0u;
+  total = sh[BT / 64 - 1];
+  __syncthreads();
+  return before + incl - v;
+}
+
+}  // namespace dtlh

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libdtl_hip.so")
 OK, EINVAL, ERUNTIME, ELOGIC, ENOMEM, EHIP, ENODEVICE = 0, -1, -2, -3, -4, -5, -6
 ADDR_POW2, ADDR_MAGIC = 0, 1
 PROBE_FORCE_GLOBAL = 1
+PROBE_PARTITIONED = 2
+PROBE_DIRECT = 4
 
 
 class BbfDesc(C.Structure):
@@ -37,7 +39,8 @@ class CfDesc(C.Structure):
 class ProbeParams(C.Structure):
     _fields_ = [("lds_max_bytes", C.c_uint32), ("block_threads", C.c_uint32),
                 ("grid_blocks", C.c_uint32), ("chunk_tiles", C.c_uint32), ("flags", C.c_uint32),
-                ("ev_start", C.c_void_p), ("ev_stop", C.c_void_p)]
+                ("ev_start", C.c_void_p), ("ev_stop", C.c_void_p), ("slice_log2", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class BbfTask(C.Structure):

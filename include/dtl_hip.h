@@ -98,15 +98,28 @@ typedef struct dtl_cf_desc {
 /* Tuning knobs of one probe launch (0 = library default). */
 typedef struct dtl_probe_params {
   uint32_t lds_max_bytes;   /* filters up to this size are staged in LDS (default: device LDS limit) */
-  uint32_t block_threads;   /* threads per workgroup of the persistent kernels (256 or 1024) */
+  uint32_t block_threads;   /* threads per workgroup of the persistent kernels (256 or 1024);
+                               partitioned probe: of the key scatter (256, 512 or 1024) */
   uint32_t grid_blocks;     /* workgroups of the persistent kernels (default: CUs x occupancy) */
   uint32_t chunk_tiles;     /* batched: tiles of 2048 keys a workgroup takes per grab (default 1) */
-  uint32_t flags;           /* DTL_PROBE_FORCE_GLOBAL: never stage in LDS */
-  void* ev_start;           /* optional hipEvent_t recorded right before the probe kernel */
+  uint32_t flags;           /* DTL_PROBE_* below */
+  void* ev_start;           /* optional hipEvent_t recorded right before the probe phase */
   void* ev_stop;            /* optional hipEvent_t recorded right after it (before compaction) */
+  uint32_t slice_log2;      /* partitioned probe: log2 of the filter bytes per partition
+                               (default 21, 22 for filters of 512 MiB or more) */
+  uint32_t reserved;
 } dtl_probe_params;
 
-enum { DTL_PROBE_FORCE_GLOBAL = 1u };
+/* Probe strategy.  By default filters that fit in LDS are staged there; larger
+ * filters of at least 64 MiB probed with at least 2^22 keys use the
+ * partitioned probe (keys grouped by filter slice so that each slice is read
+ * into L2 once); everything else reads blocks straight from memory.  The
+ * result is the same list in every case. */
+enum {
+  DTL_PROBE_FORCE_GLOBAL = 1u,   /* never stage in LDS */
+  DTL_PROBE_PARTITIONED = 2u,    /* always use the partitioned probe (filters larger than LDS) */
+  DTL_PROBE_DIRECT = 4u          /* never use the partitioned probe */
+};
 
 /* ---- descriptors -------------------------------------------------------- */
 /* Validates (word_bits, k, w, s, z) like the reference's runtime dispatch and

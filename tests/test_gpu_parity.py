@@ -144,6 +144,44 @@ def test_edge_sizes(n):
         assert np.array_equal(D.positions(out, cnt), exp)
 
 
+def part_params(slice_log2):
+    p = D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_PARTITIONED)
+    p.slice_log2 = slice_log2
+    return p
+
+
+PART_CASES = [
+    ((64, 1 << 22, 8, 8, 8, 1), 12, 300000),       # 512 KiB filter, 4 KiB slices -> 128 partitions
+    ((64, 1 << 25, 8, 8, 8, 1), 8, 500003),        # 4 MiB, slices widened until 2048 partitions
+    ((32, 10 * 200000, 6, 16, 16, 2), 13, 123457),  # zoned MAGIC, ragged partitions
+    ((32, 1 << 21, 4, 1, 1, 1), 10, 70000),        # register-blocked u32 w1
+    ((64, 3000011, 7, 4, 1, 1), 11, 99999),        # MAGIC, word-blocked u64 w4
+    ((32, 5000000, 16, 8, 2, 1), 14, 4097),        # sltw-style, n just over 2 tiles
+]
+
+
+@pytest.mark.parametrize("cfg,slice_log2,n_probe", PART_CASES)
+def test_partitioned_probe_matches_oracle(cfg, slice_log2, n_probe):
+    """Partitioned probe (keys grouped by filter slice) equals the oracle."""
+    a = check_cfg(cfg, 40000, n_probe, seed=31, hit=0.15, offset=77, params=part_params(slice_log2))
+    b = check_cfg(cfg, 40000, n_probe, seed=31, hit=0.15, offset=77,
+                  params=D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_DIRECT))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 2047, 2049, 4095, 4096, 4097, 10000])
+def test_partitioned_probe_small_batches(n):
+    cfg = (64, 1 << 20, 8, 8, 8, 1)
+    check_cfg(cfg, 5000, n, seed=n, hit=0.4, offset=3, params=part_params(10))
+
+
+def test_partitioned_probe_repeated_calls_reuse_scratch():
+    """Back-to-back partitioned probes of different sizes on one stream."""
+    cfg = (64, 1 << 23, 8, 8, 8, 1)
+    for n, seed in ((200000, 1), (50000, 2), (300001, 3)):
+        check_cfg(cfg, 60000, n, seed=seed, hit=0.05, params=part_params(14))
+
+
 def test_match_offset_wraps_like_u32():
     cfg = (32, 1 << 14, 4, 1, 1, 1)
     off = 2**32 - 10
