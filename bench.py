@@ -295,39 +295,39 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
     for t in tw_ns:
         row = {"plain": round(plain_ms + t * fp * 1e-6, 3), "batched": round(batch_ms + t * fp * 1e-6, 3)}
         if model is not None:
-            log("batched: learned configurations for tw=%g ns" % t)
-            choices = {}
+            for leg, use_mlp in (("learned_batched", True), ("exhaustive_batched", False)):
+                log("batched: %s configurations for tw=%g ns" % (leg, t))
+                choices = {}
 
-            def learned(f, nb):
-                ch = choices.get(nb)
-                if ch is None:
-                    ch = choices[nb] = model.choose(nb, t, use_mlp=True)
-                return ch.word_bits, ch.k, ch.w, ch.s, ch.z, ch.m
+                def chosen(f, nb):
+                    ch = choices.get(nb)
+                    if ch is None:
+                        ch = choices[nb] = model.choose(nb, t, use_mlp=use_mlp)
+                    return ch.word_bits, ch.k, ch.w, ch.s, ch.z, ch.m
 
-            lt = build_filters(torch, D, meta, probes, learned)
-            l_ms, l_counts, _, groups = time_graph_plan(torch, D, lt, steps, tune_reps)
-            l_fp = sum(l_counts) - true_hits
-            used = {}
-            for desc, _, _, _ in lt:
-                key = "u%d w%d s%d z%d k%d" % (desc.word_bits, desc.w, desc.s, desc.z, desc.k)
-                used[key] = used.get(key, 0) + 1
-            l_bytes = int(sum(x[0].size_bytes for x in lt))
-            del lt
-            torch.cuda.empty_cache()
-            row["learned_batched"] = round(l_ms + t * l_fp * 1e-6, 3)
-            row["learned_probe_ms"] = round(l_ms, 3)
-            row["learned_false_positives"] = l_fp
-            row["learned_filter_bytes"] = l_bytes
-            row["learned_config_groups"] = groups
-            row["learned_configs"] = dict(sorted(used.items(), key=lambda kv: -kv[1])[:6])
-            row["plain_over_learned_batched"] = round(row["plain"] / row["learned_batched"], 3)
+                lt = build_filters(torch, D, meta, probes, chosen)
+                l_ms, l_counts, _, groups = time_graph_plan(torch, D, lt, steps, tune_reps)
+                l_fp = sum(l_counts) - true_hits
+                used = {}
+                for desc, _, _, _ in lt:
+                    key = "u%d w%d s%d z%d k%d" % (desc.word_bits, desc.w, desc.s, desc.z, desc.k)
+                    used[key] = used.get(key, 0) + 1
+                l_bytes = int(sum(x[0].size_bytes for x in lt))
+                del lt
+                torch.cuda.empty_cache()
+                row[leg] = {"total_ms": round(l_ms + t * l_fp * 1e-6, 3), "probe_ms": round(l_ms, 3),
+                            "false_positives": l_fp, "filter_bytes": l_bytes, "config_groups": groups,
+                            "top_configs": dict(sorted(used.items(), key=lambda kv: -kv[1])[:4])}
+            row["plain_over_learned_batched"] = round(row["plain"] / row["learned_batched"]["total_ms"], 3)
+            row["learned_regret_vs_exhaustive"] = round(
+                row["learned_batched"]["total_ms"] / row["exhaustive_batched"]["total_ms"] - 1.0, 4)
         res["total_filter_cost_ms"]["tw=%gns" % t] = row
     del probes
     torch.cuda.empty_cache()
     return res
 
 
-def run_learned(torch, D, train_fraction=0.25, samples=400000, epochs=40, model_path=None):
+def run_learned(torch, D, train_fraction=0.25, samples=2000000, epochs=16, model_path=None):
     """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
     bits, timed on the GPU; FPR measured per configuration and bits per key),
     MLP trained on a seeded `train_fraction` of the (configuration, size)

@@ -58,11 +58,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ p
   y[r] = out;
 }
 
-// Gradient of 0.5 * mean((f(x) - t)^2) over the minibatch rows
-// idx(j) = (a * (base + j) + c) mod n, j < count (an epoch-specific permutation).
+// Gradient of 0.5 * mean(w * (f(x) - t)^2) over the minibatch rows
+// idx(j) = (a * (base + j) + c) mod n, j < count (an epoch-specific
+// permutation); w = 1 when no weights are given.
 __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ params,
                                                   const float* __restrict__ x,
-                                                  const float* __restrict__ t, uint32_t n,
+                                                  const float* __restrict__ t,
+                                                  const float* __restrict__ wts, uint32_t n,
                                                   uint64_t perm_a, uint64_t perm_c, uint32_t base,
                                                   uint32_t count, float* __restrict__ grad) {
   __shared__ float p[kParams];
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ para
       float yv = p[oB3];
       for (int k = 0; k < kH; ++k) yv += p[oW3 + k] * H2[tid][k];
       const uint64_t idx = (perm_a * (base + row0 + tid) + perm_c) % n;
-      dv = (yv - t[idx]) / static_cast<float>(count);
+      dv = (yv - t[idx]) * (wts ? wts[idx] : 1.0f) / static_cast<float>(count);
     }
     dy[tid] = dv;
   }
@@ -261,11 +263,12 @@ int dtl_mlp_set_params(dtl_mlp* m, const float* h_params) {
              : DTL_EHIP;
 }
 
-// Trains on n samples (d_x: n x 8 features, d_t: n targets; device memory)
-// with Adam, minibatches of `batch` rows, `epochs` passes; each epoch visits
-// the rows in the order (a * j + c) mod n for a fresh (a coprime to n, c).
-int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, uint32_t n, uint32_t epochs,
-                  uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream) {
+// Trains on n samples (d_x: n x 8 features, d_t: n targets, d_w: n sample
+// weights or null; device memory) with Adam, minibatches of `batch` rows,
+// `epochs` passes; each epoch visits the rows in the order (a * j + c) mod n
+// for a fresh (a coprime to n, c).
+int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
+                  uint32_t epochs, uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream) {
   if (!m || !d_x || !d_t || n == 0 || batch == 0) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   (void)hipGetLastError();
@@ -296,8 +299,8 @@ int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, uint32_t n, ui
     for (uint32_t b0 = 0; b0 < n; b0 += batch) {
       const uint32_t cnt = std::min(batch, n - b0);
       const uint32_t blocks = (cnt + kRows - 1) / kRows;
-      hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, st, params, d_x, d_t, n, a, c, b0, cnt,
-                         grad);
+      hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, st, params, d_x, d_t, d_w, n, a, c, b0,
+                         cnt, grad);
       m->step += 1;
       const float b1t = std::pow(0.9f, static_cast<float>(m->step));
       const float b2t = std::pow(0.999f, static_cast<float>(m->step));
@@ -313,15 +316,16 @@ int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, uint32_t n, ui
   return DTL_OK;
 }
 
-// Gradient of 0.5 * mean((f(x) - t)^2) over all n rows into d_grad (kParams
-// floats, device); the model's parameters are unchanged.
-int dtl_mlp_gradient(const dtl_mlp* m, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
-                     dtl_stream_t stream) {
+// Gradient of 0.5 * mean(w * (f(x) - t)^2) over all n rows (w = d_w[i], or
+// 1 if d_w is null) into d_grad (kParams floats, device); the model's
+// parameters are unchanged.
+int dtl_mlp_gradient(const dtl_mlp* m, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
+                     float* d_grad, dtl_stream_t stream) {
   if (!m || !d_x || !d_t || !d_grad || n == 0) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   (void)hipGetLastError();
   hipLaunchKernelGGL(k_fill_zero, dim3((kParams + 255) / 256), dim3(256), 0, st, d_grad, kParams);
-  hipLaunchKernelGGL(k_mlp_grad, dim3((n + kRows - 1) / kRows), dim3(256), 0, st, m->d, d_x, d_t, n,
+  hipLaunchKernelGGL(k_mlp_grad, dim3((n + kRows - 1) / kRows), dim3(256), 0, st, m->d, d_x, d_t, d_w, n,
                      uint64_t(1), uint64_t(0), 0u, n, d_grad);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) {

@@ -10,7 +10,7 @@
 //     log2(m) -- the GPU analogue of the reference's per-memory-level timings
 //     (model/calibration.cpp:251-311, benchmark.cpp:104-199);
 //   * fpr: measured on the GPU for every configuration at 8 bits-per-key
-//     points (n = 2^16 built keys, 2^22 absent keys probed), interpolated in
+//     points (n = 2^16 built keys, 2^24 absent keys probed), interpolated in
 //     log(fpr) over bits per key.
 // Candidate space: the reference's valid configurations (model/util.hpp:16-63:
 // word size x w <= 64 B, s <= w, k % s == 0; zoned w >= 4) with z in {1,2,4},
@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "dtl_hip.h"
@@ -111,7 +112,7 @@ void features(const Cfg& c, double m, double n, double tw, float* f) {
   f[3] = static_cast<float>(std::log2(c.z) / 2.0);
   f[4] = c.k / 16.0f;
   f[5] = static_cast<float>((std::log2(m) - 23.0) / 10.0);
-  f[6] = static_cast<float>((std::log2(n) - 19.0) / 9.0);
+  f[6] = static_cast<float>(std::log2(m / n) - 3.3);  // bits per key
   f[7] = static_cast<float>(std::log2(tw) / 10.0);
 }
 
@@ -153,7 +154,7 @@ int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_str
   const uint32_t lmax = (p && p->log2_m_max) ? p->log2_m_max : 33;
   const uint32_t lkeys = (p && p->probe_keys_log2) ? p->probe_keys_log2 : 24;
   const uint32_t reps = (p && p->reps) ? p->reps : 3;
-  const uint32_t lfpr = (p && p->fpr_keys_log2) ? p->fpr_keys_log2 : 22;
+  const uint32_t lfpr = (p && p->fpr_keys_log2) ? p->fpr_keys_log2 : 24;
   if (lmin < 10 || lmax > 34 || lmin > lmax || lkeys > 30) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const auto t0 = std::chrono::steady_clock::now();
@@ -274,9 +275,73 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 }
 
 // Trains the MLP on a seeded `train_fraction` of the calibrated (config,
-// size) cells: `samples` rows, each a random cell with bits per key
-// log-uniform in [4, 24] and tw log-uniform in [1, 1000] ns, target
-// log(overhead) standardised.
+// size) cells (the rest of the table is never seen by the MLP).  Those cells
+// are split 80/20 into fitting and validation cells.  Rows: a random cell,
+// bits per key log-uniform in [4, 20] (the candidate range) and tw
+// log-uniform in [1, 1000] ns, target y = log(overhead) standardised.  The
+// loss weights a row by exp(-(y - y_best) / 0.1) + 0.2, y_best = the lowest
+// y of any training cell at the same (size, bits per key, tw): the choice
+// only depends on the ranking near the optimum, so that is where the fit has
+// to be tight.  kRestarts networks (different initialisations) are trained
+// and the one with the lowest weighted error on the validation rows is kept.
+constexpr int kRestarts = 4;
+
+namespace {
+
+struct Rows {
+  std::vector<float> X, Y, W;
+};
+
+// Draws `count` rows from `cells`; y_best over every cell in `ref`.
+void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std::vector<uint8_t>& ref,
+               uint32_t count, uint64_t& s, Rows& r) {
+  const int nc = static_cast<int>(M.cfgs.size()), ns = M.nsizes();
+  auto next = [&]() {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return s;
+  };
+  auto unif = [&]() { return (next() >> 11) * (1.0 / 9007199254740992.0); };
+  r.X.assign(static_cast<size_t>(count) * 8, 0.f);
+  r.Y.assign(count, 0.f);
+  r.W.assign(count, 0.f);
+  std::vector<int> s_si(count);
+  std::vector<double> s_bpk(count), s_tw(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const int cell = cells[next() % cells.size()];
+    const int c = cell / ns, si = cell % ns;
+    const double m = std::ldexp(1.0, M.log2_min + si);
+    const double bpk = 4.0 * std::pow(5.0, unif());
+    const double tw = std::pow(1000.0, unif());
+    features(M.cfgs[c], m, m / bpk, tw, &r.X[static_cast<size_t>(i) * 8]);
+    r.Y[i] = static_cast<float>(std::log(M.t_ns[cell] + tw * interp_fpr(M, c, bpk)));
+    s_si[i] = si;
+    s_bpk[i] = bpk;
+    s_tw[i] = tw;
+  }
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t)
+    th.emplace_back([&, t]() {
+      for (uint32_t i = t; i < count; i += nth) {
+        double best = 1e300;
+        for (int c = 0; c < nc; ++c) {
+          const size_t cell = static_cast<size_t>(c) * ns + s_si[i];
+          if (ref[cell]) best = std::min(best, M.t_ns[cell] + s_tw[i] * interp_fpr(M, c, s_bpk[i]));
+        }
+        r.W[i] = static_cast<float>(std::exp(-(r.Y[i] - std::log(best)) / 0.1) + 0.2);
+      }
+    });
+  for (auto& x : th) x.join();
+  double wsum = 0;
+  for (float w : r.W) wsum += w;
+  const float scale = static_cast<float>(count / wsum);
+  for (float& w : r.W) w *= scale;
+}
+
+}  // namespace
+
 int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, uint32_t epochs,
                     uint32_t seed, dtl_stream_t stream) {
   if (!M || samples == 0 || train_fraction <= 0 || train_fraction > 1) return DTL_EINVAL;
@@ -290,50 +355,70 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     return s;
   };
   auto unif = [&]() { return (next() >> 11) * (1.0 / 9007199254740992.0); };
-  std::vector<int> cells;
+  std::vector<int> fit_cells, val_cells;
   for (int i = 0; i < nc * ns; ++i) {
     M->train_cell[i] = unif() < train_fraction ? 1 : 0;
-    if (M->train_cell[i]) cells.push_back(i);
+    if (M->train_cell[i]) (unif() < 0.8 ? fit_cells : val_cells).push_back(i);
   }
-  if (cells.empty()) return DTL_EINVAL;
-  std::vector<float> X(static_cast<size_t>(samples) * 8), Y(samples);
+  if (fit_cells.empty()) return DTL_EINVAL;
+  if (val_cells.empty()) val_cells = fit_cells;
+  Rows fit, val;
+  draw_rows(*M, fit_cells, M->train_cell, samples, s, fit);
+  draw_rows(*M, val_cells, M->train_cell, std::max(1u, samples / 8), s, val);
   double sum = 0, sum2 = 0;
-  for (uint32_t i = 0; i < samples; ++i) {
-    const int cell = cells[next() % cells.size()];
-    const int c = cell / ns, si = cell % ns;
-    const double m = std::ldexp(1.0, M->log2_min + si);
-    const double bpk = 4.0 * std::pow(6.0, unif());
-    const double tw = std::pow(1000.0, unif());
-    const double n = m / bpk;
-    const double y = std::log(M->t_ns[cell] + tw * interp_fpr(*M, c, bpk));
-    features(M->cfgs[c], m, n, tw, &X[static_cast<size_t>(i) * 8]);
-    Y[i] = static_cast<float>(y);
+  for (float y : fit.Y) {
     sum += y;
-    sum2 += y * y;
+    sum2 += static_cast<double>(y) * y;
   }
   M->y_mean = sum / samples;
   M->y_std = std::sqrt(std::max(1e-12, sum2 / samples - M->y_mean * M->y_mean));
-  for (auto& y : Y) y = static_cast<float>((y - M->y_mean) / M->y_std);
-  DevBuf dx, dy;
-  if (!dx.alloc(X.size() * 4) || !dy.alloc(Y.size() * 4)) return DTL_ENOMEM;
-  if (hipMemcpyAsync(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(dy.p, Y.data(), Y.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+  for (auto* y : {&fit.Y, &val.Y})
+    for (auto& v : *y) v = static_cast<float>((v - M->y_mean) / M->y_std);
+  const uint32_t nval = static_cast<uint32_t>(val.Y.size());
+  DevBuf dx, dy, dw, dvx, dvy;
+  if (!dx.alloc(fit.X.size() * 4) || !dy.alloc(fit.Y.size() * 4) || !dw.alloc(fit.W.size() * 4) ||
+      !dvx.alloc(val.X.size() * 4) || !dvy.alloc(nval * 4))
+    return DTL_ENOMEM;
+  if (hipMemcpyAsync(dx.p, fit.X.data(), fit.X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dy.p, fit.Y.data(), fit.Y.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dw.p, fit.W.data(), fit.W.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dvx.p, val.X.data(), val.X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
     return DTL_EHIP;
+  const float* px = static_cast<float*>(dx.p);
+  const float* py = static_cast<float*>(dy.p);
+  const float* pw = static_cast<float*>(dw.p);
+  const uint32_t ep = epochs ? epochs : 16;
   if (M->mlp) dtl_mlp_destroy(M->mlp);
   M->mlp = nullptr;
-  int rc = dtl_mlp_create(seed, &M->mlp);
+  std::vector<float> best_params(dtl_mlp_param_count()), pred(nval);
+  double best_err = 1e300;
+  int rc = DTL_OK;
+  for (int rsi = 0; rsi < kRestarts && rc == DTL_OK; ++rsi) {
+    dtl_mlp* net = nullptr;
+    const uint32_t sd = seed * 7919u + static_cast<uint32_t>(rsi) * 104729u;
+    rc = dtl_mlp_create(sd, &net);
+    if (rc != DTL_OK) break;
+    // learning-rate schedule: 3 phases
+    rc = dtl_mlp_train(net, px, py, pw, samples, ep / 2 + 1, 512, 2e-3f, sd, stream);
+    if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 4 + 1, 1024, 5e-4f, sd + 1, stream);
+    if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 8 + 1, 2048, 1e-4f, sd + 2, stream);
+    if (rc == DTL_OK) rc = dtl_mlp_predict(net, static_cast<float*>(dvx.p), nval, static_cast<float*>(dvy.p), stream);
+    if (rc == DTL_OK && (hipMemcpyAsync(pred.data(), dvy.p, nval * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                         hipStreamSynchronize(st) != hipSuccess))
+      rc = DTL_EHIP;
+    if (rc == DTL_OK) {
+      double err = 0;
+      for (uint32_t i = 0; i < nval; ++i) err += val.W[i] * (pred[i] - val.Y[i]) * (pred[i] - val.Y[i]);
+      if (err < best_err) {
+        best_err = err;
+        rc = dtl_mlp_get_params(net, best_params.data());
+      }
+    }
+    dtl_mlp_destroy(net);
+  }
   if (rc != DTL_OK) return rc;
-  const uint32_t ep = epochs ? epochs : 40;
-  // learning-rate schedule: 3 phases
-  rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 2 + 1, 512,
-                     2e-3f, seed, stream);
-  if (rc == DTL_OK)
-    rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 3 + 1,
-                       1024, 5e-4f, seed + 1, stream);
-  if (rc == DTL_OK)
-    rc = dtl_mlp_train(M->mlp, static_cast<float*>(dx.p), static_cast<float*>(dy.p), samples, ep / 6 + 1,
-                       2048, 1e-4f, seed + 2, stream);
-  if (rc == DTL_OK && hipStreamSynchronize(st) != hipSuccess) rc = DTL_EHIP;
+  rc = dtl_mlp_create(seed, &M->mlp);
+  if (rc == DTL_OK) rc = dtl_mlp_set_params(M->mlp, best_params.data());
   return rc;
 }
 

@@ -146,9 +146,9 @@ def load(path=LIB_PATH):
     L.dtl_mlp_create.argtypes = [u32, C.POINTER(vp)]
     L.dtl_mlp_destroy.argtypes = [vp]
     L.dtl_mlp_destroy.restype = None
-    L.dtl_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, f32, u32, vp]
+    L.dtl_mlp_train.argtypes = [vp, vp, vp, vp, u32, u32, u32, f32, u32, vp]
     L.dtl_mlp_predict.argtypes = [vp, vp, u32, vp, vp]
-    L.dtl_mlp_gradient.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.dtl_mlp_gradient.argtypes = [vp, vp, vp, vp, u32, vp, vp]
     L.dtl_mlp_get_params.argtypes = [vp, C.POINTER(C.c_float)]
     L.dtl_mlp_set_params.argtypes = [vp, C.POINTER(C.c_float)]
     for name in ("dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_config_count",
@@ -346,16 +346,17 @@ class MLP:
         _check("dtl_mlp_predict", self.L.dtl_mlp_predict(self.h, _ptr(x), x.shape[0], _ptr(y), _stream(None)))
         return y
 
-    def gradient(self, x, t):
+    def gradient(self, x, t, w=None):
         import torch
         g = torch.empty(self.n_params, dtype=torch.float32, device=x.device)
-        _check("dtl_mlp_gradient", self.L.dtl_mlp_gradient(self.h, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
-                                                           _stream(None)))
+        _check("dtl_mlp_gradient", self.L.dtl_mlp_gradient(self.h, _ptr(x), _ptr(t),
+                                                           _ptr(w) if w is not None else None,
+                                                           x.shape[0], _ptr(g), _stream(None)))
         return g
 
-    def train(self, x, t, epochs, batch, lr, seed=0):
-        _check("dtl_mlp_train", self.L.dtl_mlp_train(self.h, _ptr(x), _ptr(t), x.shape[0], epochs, batch,
-                                                     lr, seed, _stream(None)))
+    def train(self, x, t, epochs, batch, lr, seed=0, w=None):
+        _check("dtl_mlp_train", self.L.dtl_mlp_train(self.h, _ptr(x), _ptr(t), _ptr(w) if w is not None else None,
+                                                     x.shape[0], epochs, batch, lr, seed, _stream(None)))
 
     def close(self):
         if self.h:

@@ -209,7 +209,7 @@ typedef struct dtl_calib_params {
   uint32_t log2_m_max;      /* largest, log2 bits (default 33 = 1 GiB) */
   uint32_t probe_keys_log2; /* keys per timing run (default 24) */
   uint32_t reps;            /* timed repetitions, median kept (default 3) */
-  uint32_t fpr_keys_log2;   /* absent keys per FPR measurement (default 22) */
+  uint32_t fpr_keys_log2;   /* absent keys per FPR measurement (default 24) */
   uint32_t flags;
 } dtl_calib_params;
 
@@ -242,11 +242,13 @@ typedef struct dtl_mlp dtl_mlp;
 int dtl_mlp_param_count(void);
 int dtl_mlp_create(uint32_t seed, dtl_mlp** mlp);
 void dtl_mlp_destroy(dtl_mlp* mlp);
-int dtl_mlp_train(dtl_mlp* mlp, const float* d_x, const float* d_t, uint32_t n, uint32_t epochs,
-                  uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream);
+/* Adam on 0.5 * mean(w * (f(x) - t)^2); d_x: n x 8 features, d_t: n targets,
+ * d_w: n sample weights or NULL (all 1); device memory. */
+int dtl_mlp_train(dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
+                  uint32_t epochs, uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream);
 int dtl_mlp_predict(const dtl_mlp* mlp, const float* d_x, uint32_t n, float* d_y, dtl_stream_t stream);
-int dtl_mlp_gradient(const dtl_mlp* mlp, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
-                     dtl_stream_t stream);
+int dtl_mlp_gradient(const dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
+                     float* d_grad, dtl_stream_t stream);
 int dtl_mlp_get_params(const dtl_mlp* mlp, float* h_params);
 int dtl_mlp_set_params(dtl_mlp* mlp, const float* h_params);
 
