@@ -275,15 +275,16 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 }
 
 // Trains the MLP on a seeded `train_fraction` of the calibrated (config,
-// size) cells (the rest of the table is never seen by the MLP).  Those cells
-// are split 80/20 into fitting and validation cells.  Rows: a random cell,
+// size) cells (the rest of the table is never seen by the MLP).  Rows: a random cell,
 // bits per key log-uniform in [4, 20] (the candidate range) and tw
 // log-uniform in [1, 1000] ns, target y = log(overhead) standardised.  The
 // loss weights a row by exp(-(y - y_best) / 0.1) + 0.2, y_best = the lowest
 // y of any training cell at the same (size, bits per key, tw): the choice
 // only depends on the ranking near the optimum, so that is where the fit has
 // to be tight.  kRestarts networks (different initialisations) are trained
-// and the one with the lowest weighted error on the validation rows is kept.
+// and the one whose choices have the lowest mean regret on a query grid,
+// judged on the training cells only (candidates at calibrated sizes, table
+// cost of those cells), is kept.
 constexpr int kRestarts = 4;
 
 namespace {
@@ -355,16 +356,14 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     return s;
   };
   auto unif = [&]() { return (next() >> 11) * (1.0 / 9007199254740992.0); };
-  std::vector<int> fit_cells, val_cells;
+  std::vector<int> fit_cells;
   for (int i = 0; i < nc * ns; ++i) {
     M->train_cell[i] = unif() < train_fraction ? 1 : 0;
-    if (M->train_cell[i]) (unif() < 0.8 ? fit_cells : val_cells).push_back(i);
+    if (M->train_cell[i]) fit_cells.push_back(i);
   }
   if (fit_cells.empty()) return DTL_EINVAL;
-  if (val_cells.empty()) val_cells = fit_cells;
-  Rows fit, val;
+  Rows fit;
   draw_rows(*M, fit_cells, M->train_cell, samples, s, fit);
-  draw_rows(*M, val_cells, M->train_cell, std::max(1u, samples / 8), s, val);
   double sum = 0, sum2 = 0;
   for (float y : fit.Y) {
     sum += y;
@@ -372,17 +371,41 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
   }
   M->y_mean = sum / samples;
   M->y_std = std::sqrt(std::max(1e-12, sum2 / samples - M->y_mean * M->y_mean));
-  for (auto* y : {&fit.Y, &val.Y})
-    for (auto& v : *y) v = static_cast<float>((v - M->y_mean) / M->y_std);
-  const uint32_t nval = static_cast<uint32_t>(val.Y.size());
+  for (auto& v : fit.Y) v = static_cast<float>((v - M->y_mean) / M->y_std);
+  // selection queries: n = 2^10 .. 2^28, tw = 1 .. 1000 ns; candidates are
+  // the training cells whose size gives 4..20 bits per key
+  struct Query {
+    uint32_t begin, end;
+  };
+  std::vector<Query> queries;
+  std::vector<float> qx;
+  std::vector<double> qcost;
+  for (int ln = 10; ln <= 28; ln += 2)
+    for (double tw : {1.0, 3.0, 10.0, 30.0, 100.0, 300.0, 1000.0}) {
+      Query q{static_cast<uint32_t>(qcost.size()), 0};
+      const double n = std::ldexp(1.0, ln);
+      for (int c = 0; c < nc; ++c)
+        for (int si = 0; si < ns; ++si) {
+          if (!M->train_cell[static_cast<size_t>(c) * ns + si]) continue;
+          const double m = std::ldexp(1.0, M->log2_min + si), bpk = m / n;
+          if (bpk < 4.0 || bpk > 20.0) continue;
+          float f[8];
+          features(M->cfgs[c], m, n, tw, f);
+          qx.insert(qx.end(), f, f + 8);
+          qcost.push_back(M->t_ns[static_cast<size_t>(c) * ns + si] + tw * interp_fpr(*M, c, bpk));
+        }
+      q.end = static_cast<uint32_t>(qcost.size());
+      if (q.end > q.begin + 1) queries.push_back(q);
+    }
+  const uint32_t nq = static_cast<uint32_t>(std::max<size_t>(qcost.size(), 1));
   DevBuf dx, dy, dw, dvx, dvy;
   if (!dx.alloc(fit.X.size() * 4) || !dy.alloc(fit.Y.size() * 4) || !dw.alloc(fit.W.size() * 4) ||
-      !dvx.alloc(val.X.size() * 4) || !dvy.alloc(nval * 4))
+      !dvx.alloc(static_cast<size_t>(nq) * 32) || !dvy.alloc(static_cast<size_t>(nq) * 4))
     return DTL_ENOMEM;
   if (hipMemcpyAsync(dx.p, fit.X.data(), fit.X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dy.p, fit.Y.data(), fit.Y.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dw.p, fit.W.data(), fit.W.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(dvx.p, val.X.data(), val.X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+      (!qx.empty() && hipMemcpyAsync(dvx.p, qx.data(), qx.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess))
     return DTL_EHIP;
   const float* px = static_cast<float*>(dx.p);
   const float* py = static_cast<float*>(dy.p);
@@ -390,7 +413,7 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
   const uint32_t ep = epochs ? epochs : 16;
   if (M->mlp) dtl_mlp_destroy(M->mlp);
   M->mlp = nullptr;
-  std::vector<float> best_params(dtl_mlp_param_count()), pred(nval);
+  std::vector<float> best_params(dtl_mlp_param_count()), pred(nq);
   double best_err = 1e300;
   int rc = DTL_OK;
   for (int rsi = 0; rsi < kRestarts && rc == DTL_OK; ++rsi) {
@@ -402,13 +425,24 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     rc = dtl_mlp_train(net, px, py, pw, samples, ep / 2 + 1, 512, 2e-3f, sd, stream);
     if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 4 + 1, 1024, 5e-4f, sd + 1, stream);
     if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 8 + 1, 2048, 1e-4f, sd + 2, stream);
-    if (rc == DTL_OK) rc = dtl_mlp_predict(net, static_cast<float*>(dvx.p), nval, static_cast<float*>(dvy.p), stream);
-    if (rc == DTL_OK && (hipMemcpyAsync(pred.data(), dvy.p, nval * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    const uint32_t npred = static_cast<uint32_t>(qcost.size());
+    if (rc == DTL_OK && npred)
+      rc = dtl_mlp_predict(net, static_cast<float*>(dvx.p), npred, static_cast<float*>(dvy.p), stream);
+    if (rc == DTL_OK && (hipMemcpyAsync(pred.data(), dvy.p, static_cast<size_t>(npred) * 4,
+                                        hipMemcpyDeviceToHost, st) != hipSuccess ||
                          hipStreamSynchronize(st) != hipSuccess))
       rc = DTL_EHIP;
     if (rc == DTL_OK) {
-      double err = 0;
-      for (uint32_t i = 0; i < nval; ++i) err += val.W[i] * (pred[i] - val.Y[i]) * (pred[i] - val.Y[i]);
+      double err = 0;  // mean regret of the network's choices on the selection queries
+      for (const Query& q : queries) {
+        uint32_t pick = q.begin;
+        double best = qcost[q.begin];
+        for (uint32_t i = q.begin; i < q.end; ++i) {
+          if (pred[i] < pred[pick]) pick = i;
+          best = std::min(best, qcost[i]);
+        }
+        err += qcost[pick] / best - 1.0;
+      }
       if (err < best_err) {
         best_err = err;
         rc = dtl_mlp_get_params(net, best_params.data());
