@@ -122,9 +122,9 @@ GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal"
                 4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector"}
 
 
-def gather_table(torch, D, modes, n=1 << 28, reps=5):
-    """Random-sector read microbenchmark over a 1 GiB buffer (dtl_gather_bench)."""
-    buf = torch.randint(0, 2**31, (1 << 28,), dtype=torch.int32, device="cuda")
+def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
+    """Random-sector read microbenchmark over a 2^log2_bytes buffer (dtl_gather_bench)."""
+    buf = torch.randint(0, 2**31, (1 << (log2_bytes - 2),), dtype=torch.int32, device="cuda")
     sink = torch.zeros(1, dtype=torch.int64, device="cuda")
     out = {}
     for mode in modes:
@@ -374,6 +374,81 @@ def run_learned(torch, D, train_fraction=0.25, samples=2000000, epochs=16, model
     return res, M
 
 
+def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, load=0.95):
+    """Cuckoo filter, 16-bit tags, 4 slots per bucket, 2^26 buckets (512 MiB +
+    victim cache), loaded to 95% with distinct uniform keys by the host build
+    (cf_build_host, the reference's insert with a seeded mt19937 for kicks);
+    GPU probe of 2^28 keys at 1% and 50% hits."""
+    import numpy as np
+    desc = D.cf_desc((1 << log2_buckets) * 64, 16, 4)
+    n_build = int(load * (1 << log2_buckets) * 4)
+    log("cuckoo: generating %d distinct keys" % n_build)
+    M = (1 << 31) - 1
+    x = (np.arange(n_build, dtype=np.uint64) + np.uint64(9 << 20)) & np.uint64(M)
+    x = (x * np.uint64(0x5851F42D)) & np.uint64(M)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x2C1B3C6D)) & np.uint64(M)
+    x ^= x >> np.uint64(13)
+    x = (x * np.uint64(0x297A2D39)) & np.uint64(M)
+    keys = x.astype(np.uint32)
+    del x
+    log("cuckoo: host build")
+    t0 = time.time()
+    buf, inserted = D.cf_build_host(desc, keys, seed=1)
+    build_s = time.time() - t0
+    dbuf = torch.from_numpy(buf).cuda()
+    build_dev = torch.from_numpy(keys.view(np.int32)).cuda()
+    del keys
+    out = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    res = {"workload": "cuckoo filter batch_contains: 16-bit tags, 4 slots/bucket, 2^%d buckets (%d MiB), "
+                       "%d keys (%.0f%% load) built on the host; 2^28 probe keys" %
+                       (log2_buckets, desc.buffer_bytes >> 20, n_build, load * 100),
+           "host_build_s": round(build_s, 1), "inserted": int(inserted), "legs": []}
+    for hit in (0.01, 0.5):
+        n = 1 << 28
+        g = torch.Generator(device="cuda").manual_seed(5)
+        probe = distinct_keys(torch, n, 77, 1)
+        hits = int(round(hit * n))
+        idx = torch.randperm(n, device="cuda", generator=g)[:hits]
+        probe[idx] = build_dev[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            D.cf_probe(desc, dbuf, probe, 0, out=out, count=cnt)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            D.cf_probe(desc, dbuf, probe, 0, out=out, count=cnt)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        matches = int(cnt.item())
+        bpk = 4.0 + 2 * 64.0 + 4.0 * matches / n
+        leg = {"hit_rate": hit, "ms": round(ms, 4), "G_probes_per_s": round(n / ms / 1e6, 2),
+               "matches": matches, "false_positives": matches - hits,
+               "roofline": {"bound": "hbm", "bytes_per_probe": round(bpk, 2),
+                            "achieved": round(bpk * n / ms / 1e6, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(bpk * n / ms / 1e6 / HBM_PEAK_GBS, 4)}}
+        if not no_cpu:
+            import cpu_baseline as CB
+            import oracle as O
+            cf = O.Cuckoo(desc.m, 16, 4)
+            hk = probe[:cpu_sample].cpu().numpy().view(np.uint32).copy()
+            threads = min(CB.physical_cores(), 16)
+            simd = CB.cf_simd_available(cf)
+            best = None
+            for _ in range(3):
+                _, secs, _ = CB.cf_probe(cf, buf, hk, threads, simd)
+                best = secs if best is None else min(best, secs)
+            leg["cpu_baseline"] = {"value": round(len(hk) / best / 1e9, 4), "unit": "G probes/s", "cores": threads,
+                                   "kind": "port", "sample": "%d probe keys, %s, best of 3" %
+                                   (len(hk), "AVX-512" if simd else "scalar")}
+        res["legs"].append(leg)
+        del probe
+    return res
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -397,6 +472,8 @@ def main():
     ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
     ap.add_argument("--batched", type=int, default=0, metavar="FILTERS",
                     help="only run the batched semi-join workload with this many filters (4096 = full)")
+    ap.add_argument("--cuckoo", action="store_true",
+                    help="only run the cuckoo filter workload (2^26 buckets at 95%% load, host build)")
     ap.add_argument("--learned", action="store_true",
                     help="only run the learned configuration model (calibration, training, regret); with "
                          "--batched also the learned + batched semi-join")
@@ -415,6 +492,10 @@ def main():
     D.load()
     torch.cuda.set_device(0)
 
+    if args.cuckoo:
+        print(json.dumps({"cuckoo": run_cuckoo(torch, D, args.steps, args.warmup, args.cpu_sample // 4,
+                                               args.no_cpu)}), flush=True)
+        return
     if args.learned:
         res, model = run_learned(torch, D, train_fraction=args.train_fraction, model_path=args.model_path)
         print(json.dumps({"learned_config": res}), flush=True)
@@ -430,7 +511,11 @@ def main():
         return
 
     if args.gather_only:
-        print(json.dumps({"random_sector_gather_1GiB": gather_table(torch, D, range(6))}), flush=True)
+        res = {"random_sector_gather_1GiB": gather_table(torch, D, range(6))}
+        for lb in (21, 22, 24, 27, 30):
+            res["by_size_%dMiB" % (1 << (lb - 20))] = {
+                m: r["G_sectors_per_s"] for m, r in gather_table(torch, D, (0, 2, 4, 5), reps=3, log2_bytes=lb).items()}
+        print(json.dumps(res), flush=True)
         return
 
     if args.calib_copy:
@@ -451,9 +536,15 @@ def main():
     ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, args.steps, args.warmup)
     n = probe.numel()
     value = n / (ms * 1e-3) / 1e9
+    partitioned = desc.size_bytes >= (64 << 20) and n >= (1 << 22)
     bytes_per_key = 4.0 + 64.0 + 1.0 / 8.0   # key + one 64 B block + match bitmap bit
     achieved = bytes_per_key * n / (kern_ms * 1e-3) / 1e9
     traffic = load_traffic("traffic_bbf_probe_u64_w8_s8_k8_1gib.json")
+    log("bench: direct kernel leg")
+    dparams = D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT)
+    d_ms, d_kern_ms, d_matches = time_probe(torch, D, desc, words, probe, max(3, args.steps // 2), 1,
+                                            params=dparams)
+    d_achieved = bytes_per_key * n / (d_kern_ms * 1e-3) / 1e9
 
     result = {
         "metric": METRIC,
@@ -483,9 +574,17 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
-            "kernel": "k_bbf_probe_global<u64,w8,s8,k8>",
+            "kernel": ("partitioned probe phase (k_part_hist, k_part_scatter, k_bbf_probe_global over the "
+                       "grouped keys, match reorder), HIP events around the phase" if partitioned
+                       else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
             "bytes_per_probe": bytes_per_key,
+            "direct_kernel": {
+                "kernel": "k_bbf_probe_global<u64,w8,s8,k8> (one random 64 B block per key from HBM)",
+                "kernel_ms": round(d_kern_ms, 4), "achieved": round(d_achieved, 1),
+                "frac": round(d_achieved / HBM_PEAK_GBS, 4),
+                "G_probes_per_s": round(n / (d_ms * 1e-3) / 1e9, 3),
+                "matches_identical": bool(d_matches == matches)},
         },
     }
     if not args.no_cpu:
@@ -499,7 +598,8 @@ def main():
     best = max(g.values(), key=lambda r: r["G_sectors_per_s"])
     result["roofline"]["measured_random_64B_gather"] = {
         "G_sectors_per_s": best["G_sectors_per_s"], "pattern": best["pattern"],
-        "probe_kernel_vs_gather": round(n / (kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
+        "direct_kernel_vs_gather": round(n / (d_kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
+        "probe_phase_vs_gather": round(n / (kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
         "note": "gfx950 fills a whole 128 B L2 line per random 64 B block (TCC_EA0_RDREQ_128B = all reads)"}
     if not args.no_regimes:
         log("bench: filter-size regimes")
