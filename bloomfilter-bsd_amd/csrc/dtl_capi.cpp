@@ -373,7 +373,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   if (n == 0) return hip_status(write_zero_count(d_count, st));
   const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
   const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
-  const size_t cnt_bytes = static_cast<size_t>(ntiles) * 4;
+  const size_t cnt_bytes = (static_cast<size_t>(ntiles) + kScanTmpWords) * 4;
   const uint32_t flags = params ? params->flags : 0u;
   uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
   if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
@@ -501,7 +501,7 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   if (n == 0) return hip_status(write_zero_count(d_count, st));
   const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
   const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
-  const size_t cnt_bytes = static_cast<size_t>(ntiles) * 4;
+  const size_t cnt_bytes = (static_cast<size_t>(ntiles) + kScanTmpWords) * 4;
   void* scratch = nullptr;
   rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
   if (rc != DTL_OK) return rc;

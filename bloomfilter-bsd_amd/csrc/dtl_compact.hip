@@ -2,7 +2,8 @@
 //
 // Phase 2 (scan): one workgroup per task scans the task's tile counts in
 // place into exclusive offsets and writes the task's match count.
-// Phase 3 (expand): one workgroup per tile; 256 threads each own 8 keys (one
+// Phase 3 (expand): one workgroup per tile (8 tiles per workgroup for single
+// lists); 256 threads each own 8 keys (one
 // byte of the bitmap), a workgroup exclusive scan of the byte popcounts gives
 // every match its rank, and positions i + match_offset are written in key
 // order: exactly the ascending selection vector of the reference
@@ -49,6 +50,71 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(uint32_t* cnt, uin
   scan_range(cnt, 0, ntiles, d_count, sh);
 }
 
+// Multi-workgroup scan for long count arrays: chunk b = kScanChunk counts.
+// Pass 1: partial[b] = sum of chunk b.  Pass 2: workgroup b adds up
+// partial[0 .. b) itself (at most kScanTmpWords chunks), scans its chunk with
+// that carry in place, and the last chunk writes the total.
+constexpr uint32_t kScanChunk = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ cnt, uint32_t ntiles,
+                                                              uint32_t* __restrict__ partial) {
+  __shared__ uint32_t sh[kScanThreads / 64];
+  const uint32_t c0 = blockIdx.x * kScanChunk;
+  uint32_t local = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint32_t idx = c0 + threadIdx.x * kScanItems + i;
+    local += idx < ntiles ? cnt[idx] : 0u;
+  }
+  uint32_t tot;
+  (void)block_excl_scan<kScanThreads>(local, tot, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_finish(uint32_t* __restrict__ cnt, uint32_t ntiles,
+                                                              const uint32_t* __restrict__ partial,
+                                                              uint64_t* __restrict__ d_count) {
+  __shared__ uint32_t sh[kScanThreads / 64];
+  uint32_t pre = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) pre += partial[j];
+  uint32_t tot;
+  (void)block_excl_scan<kScanThreads>(pre, tot, sh);
+  uint32_t carry = tot;  // sum of partial[0 .. blockIdx.x)
+  const uint32_t c0 = blockIdx.x * kScanChunk;
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint32_t idx = c0 + threadIdx.x * kScanItems + i;
+    v[i] = idx < ntiles ? cnt[idx] : 0u;
+    local += v[i];
+  }
+  uint32_t run = carry + block_excl_scan<kScanThreads>(local, tot, sh);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint32_t idx = c0 + threadIdx.x * kScanItems + i;
+    if (idx < ntiles) cnt[idx] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_count = carry + tot;
+}
+
+// Exclusive scan of tile_cnt[0 .. ntiles) in place, total to *d_count;
+// tile_cnt[ntiles .. ntiles + kScanTmpWords) is scratch.
+static hipError_t scan_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_count, hipStream_t stream) {
+  if (ntiles <= 4 * kScanChunk) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, stream, tile_cnt, ntiles, d_count);
+    return hipGetLastError();
+  }
+  const uint32_t chunks = (ntiles + kScanChunk - 1) / kScanChunk;
+  if (chunks > kScanTmpWords) return hipErrorInvalidValue;
+  uint32_t* partial = tile_cnt + ntiles;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(kScanThreads), 0, stream, tile_cnt, ntiles, partial);
+  hipLaunchKernelGGL(k_scan_finish, dim3(chunks), dim3(kScanThreads), 0, stream, tile_cnt, ntiles, partial,
+                     d_count);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_tasks(uint32_t* cnt,
                                                              const BatchTaskDev* tasks) {
   __shared__ uint32_t sh[kScanThreads / 64];
@@ -92,20 +158,30 @@ __device__ __forceinline__ void expand_tile_gather(const uint64_t* __restrict__ 
   }
 }
 
+// Tiles per workgroup of the single-list expands: up to 8 for long lists,
+// fewer for short ones so that the grid still covers every CU several times.
+static uint32_t expand_tiles(uint32_t ntiles) {
+  return ntiles >= 65536 ? 8u : ntiles >= 16384 ? 4u : ntiles >= 4096 ? 2u : 1u;
+}
+
 __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ offs,
+                                                                  uint32_t ntiles, uint32_t per_wg,
                                                                   const uint32_t* __restrict__ src,
                                                                   uint32_t* __restrict__ out) {
   __shared__ uint32_t sh[kExpandThreads / 64];
-  expand_tile_gather(bits, blockIdx.x, offs[blockIdx.x], src, out, sh);
+  const uint32_t t1 = min(ntiles, (blockIdx.x + 1) * per_wg);
+  for (uint32_t t = blockIdx.x * per_wg; t < t1; ++t) expand_tile_gather(bits, t, offs[t], src, out, sh);
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ offs,
+                                                                  uint32_t ntiles, uint32_t per_wg,
                                                                   uint32_t match_offset,
                                                                   uint32_t* __restrict__ pos) {
   __shared__ uint32_t sh[kExpandThreads / 64];
-  expand_tile(bits, blockIdx.x, blockIdx.x, offs[blockIdx.x], match_offset, pos, sh);
+  const uint32_t t1 = min(ntiles, (blockIdx.x + 1) * per_wg);
+  for (uint32_t t = blockIdx.x * per_wg; t < t1; ++t) expand_tile(bits, t, t, offs[t], match_offset, pos, sh);
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
@@ -126,23 +202,23 @@ hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t nti
                           uint32_t match_offset, uint32_t* pos, uint64_t* d_count,
                           hipStream_t stream) {
   (void)n;
-  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, stream, tile_cnt, ntiles,
-                     d_count);
-  hipError_t e = hipGetLastError();
+  hipError_t e = scan_counts(tile_cnt, ntiles, d_count, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_expand_single, dim3(ntiles), dim3(kExpandThreads), 0, stream, bits,
-                     tile_cnt, match_offset, pos);
+  const uint32_t per = expand_tiles(ntiles);
+  hipLaunchKernelGGL(k_expand_single, dim3((ntiles + per - 1) / per), dim3(kExpandThreads), 0, stream, bits,
+                     tile_cnt, ntiles, per, match_offset, pos);
   return hipGetLastError();
 }
 
 hipError_t scan_tile_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_total, hipStream_t stream) {
-  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, stream, tile_cnt, ntiles, d_total);
-  return hipGetLastError();
+  return scan_counts(tile_cnt, ntiles, d_total, stream);
 }
 
 hipError_t expand_gather(const uint64_t* bits, const uint32_t* offs, uint32_t ntiles, const uint32_t* src,
                          uint32_t* out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_expand_gather, dim3(ntiles), dim3(kExpandThreads), 0, stream, bits, offs, src, out);
+  const uint32_t per = expand_tiles(ntiles);
+  hipLaunchKernelGGL(k_expand_gather, dim3((ntiles + per - 1) / per), dim3(kExpandThreads), 0, stream, bits, offs,
+                     ntiles, per, src, out);
   return hipGetLastError();
 }
 

@@ -160,7 +160,10 @@ DTL_DECLARE_REG(dtl_reg_z64_w16)
 #undef DTL_DECLARE_REG
 
 // Compaction (dtl_compact.hip): exclusive scan of the tile counts (in place)
-// and expansion of the bitmap into positions.
+// and expansion of the bitmap into positions.  compact_single and
+// scan_tile_counts use tile_cnt[ntiles .. ntiles + kScanTmpWords) as scratch,
+// so every single-list tile-count buffer holds ntiles + kScanTmpWords words.
+constexpr uint32_t kScanTmpWords = 1024;
 hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t ntiles, uint32_t n,
                           uint32_t match_offset, uint32_t* pos, uint64_t* d_count,
                           hipStream_t stream);
