@@ -171,8 +171,12 @@ __device__ __forceinline__ uint32_t block_sum_waves(uint32_t v, uint32_t* red) {
 // the grouped keys and the filter slices of that range are cached in its
 // own L2 one after another (grid = 8 * tiles_per_xcd).
 // ---------------------------------------------------------------------------
+#ifndef DTL_GLOBAL_UNROLL
+#define DTL_GLOBAL_UNROLL 8  // keys in flight per lane (build-time tuning knob; 8 measured fastest)
+#endif
 constexpr int kGlobalThreads = 256;
-constexpr int kGlobalUnroll = 4;
+constexpr int kGlobalUnroll = DTL_GLOBAL_UNROLL;
+static_assert(kTileKeys / kGlobalThreads % kGlobalUnroll == 0, "rounds per tile must be a multiple of U");
 
 template <class C>
 __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
