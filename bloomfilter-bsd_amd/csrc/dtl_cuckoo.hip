@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kGlobalThreads) void k_cf_probe(const uint8_t* __re
                                                              uint32_t* __restrict__ tile_cnt) {
   constexpr int BT = kGlobalThreads;
   constexpr int ROUNDS = kTileKeys / BT;
-  constexpr int U = 4;
+  constexpr int U = kGlobalUnroll;  // keys in flight per lane
   constexpr uint32_t kMask = static_cast<uint32_t>((1ull << T) - 1ull);
   __shared__ uint32_t red[BT / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
