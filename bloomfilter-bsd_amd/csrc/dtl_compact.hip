@@ -187,11 +187,14 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t
 __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
                                                                  const uint32_t* __restrict__ offs,
                                                                  const BatchTaskDev* __restrict__ tasks,
-                                                                 const uint32_t* __restrict__ tile_task) {
+                                                                 const uint32_t* __restrict__ tile_task,
+                                                                 uint32_t total_tiles, uint32_t per_wg) {
   __shared__ uint32_t sh[kExpandThreads / 64];
-  const uint32_t g = blockIdx.x;
-  const BatchTaskDev& t = tasks[tile_task[g]];
-  expand_tile(bits, g, g - t.tile_begin, offs[g], t.match_offset, t.pos, sh);
+  const uint32_t g1 = min(total_tiles, (blockIdx.x + 1) * per_wg);
+  for (uint32_t g = blockIdx.x * per_wg; g < g1; ++g) {
+    const BatchTaskDev& t = tasks[tile_task[g]];
+    expand_tile(bits, g, g - t.tile_begin, offs[g], t.match_offset, t.pos, sh);
+  }
 }
 
 __global__ void k_zero_count(uint64_t* d_count) { *d_count = 0; }
@@ -228,8 +231,9 @@ hipError_t compact_tasks(const uint64_t* bits, uint32_t* tile_cnt, const BatchTa
   hipLaunchKernelGGL(k_scan_tasks, dim3(n_tasks), dim3(kScanThreads), 0, stream, tile_cnt, tasks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || total_tiles == 0) return e;
-  hipLaunchKernelGGL(k_expand_tasks, dim3(total_tiles), dim3(kExpandThreads), 0, stream, bits,
-                     tile_cnt, tasks, tile_task);
+  const uint32_t per = expand_tiles(total_tiles);
+  hipLaunchKernelGGL(k_expand_tasks, dim3((total_tiles + per - 1) / per), dim3(kExpandThreads), 0, stream, bits,
+                     tile_cnt, tasks, tile_task, total_tiles, per);
   return hipGetLastError();
 }
 
