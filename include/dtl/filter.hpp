@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -477,6 +478,77 @@ class cf : public filter::filter_base {
 };
 
 namespace filter {
+namespace model {
+
+// Stands in for the reference's model::calibration_data + skyline matrix
+// (model/calibration_data.hpp:34-100, skyline_matrix.hpp:118-155): the
+// GPU-calibrated cost tables and the learned MLP that choose a blocked Bloom
+// filter configuration from (n, tw).  tw is the work per qualifying tuple in
+// nanoseconds.
+class cost_model {
+  dtl_cost_model* m = nullptr;
+  explicit cost_model(dtl_cost_model* p) : m(p) {}
+
+ public:
+  cost_model(cost_model&& o) noexcept : m(o.m) { o.m = nullptr; }
+  cost_model& operator=(cost_model&& o) noexcept {
+    std::swap(m, o.m);
+    return *this;
+  }
+  cost_model(const cost_model&) = delete;
+  cost_model& operator=(const cost_model&) = delete;
+  ~cost_model() {
+    if (m) dtl_model_destroy(m);
+  }
+
+  /// Runs the calibration sweep on the current GPU (params = nullptr: the
+  /// full sweep, 2^13..2^33 bits) and, if `train`, trains the MLP on 25% of
+  /// the calibrated cells.
+  static cost_model calibrate(const dtl_calib_params* params = nullptr, bool train = true, u32 samples = 2000000,
+                              u32 epochs = 16) {
+    dtl_cost_model* p = nullptr;
+    detail::throw_status(dtl_model_calibrate(params, &p, nullptr), "calibration");
+    cost_model cm(p);
+    if (train) detail::throw_status(dtl_model_train(p, 0.25, samples, epochs, 1, nullptr), "model training");
+    return cm;
+  }
+  static cost_model load(const std::string& path) {
+    dtl_cost_model* p = nullptr;
+    if (dtl_model_load(path.c_str(), &p) != DTL_OK)
+      throw std::runtime_error("Failed to load the filter cost model from " + path + ".");
+    return cost_model(p);
+  }
+  void save(const std::string& path) const { detail::throw_status(dtl_model_save(m, path.c_str()), "model save"); }
+
+  /// Process-wide model loaded from $DTL_HIP_MODEL or ~/.dtl-hip-model.bin
+  /// (the reference reads ~/.dtl-filter.dat, calibration_data.hpp:92-100);
+  /// throws if there is none (write one with calibrate().save(path)).
+  static const cost_model& get_default_instance() {
+    static const cost_model inst = []() {
+      const char* env = std::getenv("DTL_HIP_MODEL");
+      const char* home = std::getenv("HOME");
+      const std::string path = env ? std::string(env) : std::string(home ? home : ".") + "/.dtl-hip-model.bin";
+      dtl_cost_model* p = nullptr;
+      if (dtl_model_load(path.c_str(), &p) != DTL_OK)
+        throw std::runtime_error("Failed to determine filter configuration due to missing calibration data (" +
+                                 path + ").");
+      return cost_model(p);
+    }();
+    return inst;
+  }
+
+  /// Best (configuration, m) for n keys and tw ns per qualifying tuple: the
+  /// MLP's argmin, or with use_mlp = false the calibrated table's (exhaustive).
+  dtl_bbf_choice choose(u64 n, u64 tw_ns, bool use_mlp = true) const {
+    dtl_bbf_choice ch;
+    detail::throw_status(dtl_model_choose(m, n, static_cast<double>(tw_ns), use_mlp ? 1 : 0, &ch, nullptr),
+                         "model choose");
+    return ch;
+  }
+  const dtl_cost_model* handle() const { return m; }
+};
+
+}  // namespace model
 
 // filter.hpp:35-261: owns the filter data (one device allocation instead of
 // NUMA replicas); keys and match buffers may live on the host or the device
@@ -529,6 +601,24 @@ class filter {
   }
   filter(const cuckoofilter::config& config, u64 m) : st(std::make_shared<state>()) {
     st->instance = std::make_shared<cf>(m, config.bits_per_tag, config.tags_per_bucket);
+    allocate();
+  }
+  /// filter.hpp:170-173: the configuration and size are chosen from (n, tw)
+  /// by the learned cost model (tw in ns per qualifying tuple); the time
+  /// budget is accepted and, as in the reference, not used.
+  filter(u64 n, u64 tw, u64 time_budget_millis = 1000)
+      : filter(n, tw, model::cost_model::get_default_instance(), time_budget_millis) {}
+  filter(u64 n, u64 tw, const model::cost_model& cost_model, u64 time_budget_millis = 1000)
+      : st(std::make_shared<state>()) {
+    (void)time_budget_millis;
+    const dtl_bbf_choice ch = cost_model.choose(n, tw);
+    blocked_bloomfilter_config c;
+    c.k = ch.k;
+    c.word_size = ch.word_bits / 8;
+    c.word_cnt_per_block = ch.w;
+    c.sector_cnt = ch.z > 1 ? ch.w : ch.s;
+    c.zone_cnt = ch.z;
+    st->instance = construct(c, ch.m);
     allocate();
   }
 

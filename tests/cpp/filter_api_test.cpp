@@ -140,12 +140,40 @@ static void cuckoo_test() {
   std::printf("PASS cuckoo_test\n");
 }
 
+// filter(n, tw): configuration chosen by the learned cost model (small
+// calibration grid to keep the test short), explicit model and default
+// instance loaded from $DTL_HIP_MODEL.
+static void learned_config_test() {
+  dtl_calib_params cp = {13, 22, 18, 1, 20, 0};
+  auto cm = dtl::filter::model::cost_model::calibrate(&cp, true, 200000, 8);
+  const dtl_bbf_choice ch = cm.choose(20000, 100);
+  CHECK(ch.m >= 4 * 20000ull && ch.m <= 20 * 20000ull);
+  dtl::filter::filter f(20000, 100, cm);
+  std::vector<uint32_t> keys;
+  for (uint32_t i = 1; i <= 20000; ++i) keys.push_back(i * 2654435761u);
+  CHECK(f.insert(keys.data(), keys.data() + keys.size()));
+  std::vector<uint32_t> pos(keys.size());
+  auto probe = f.probe();
+  CHECK(probe(keys.data(), keys.data() + keys.size(), pos.data(), 0) == keys.size());
+  const char* path = "/tmp/dtl_hip_model_test.bin";
+  cm.save(path);
+  setenv("DTL_HIP_MODEL", path, 1);
+  dtl::filter::filter g(20000, 100);  // default instance, loaded from the file
+  CHECK(g.instance().size_in_bytes() == f.instance().size_in_bytes());
+  CHECK(g.insert(keys.data(), keys.data() + 5000));
+  auto probe_g = g.probe();
+  CHECK(probe_g(keys.data(), keys.data() + 5000, pos.data(), 3) == 5000 && pos[0] == 3);
+  std::remove(path);
+  std::printf("PASS learned_config_test\n");
+}
+
 int main() {
   basic_build_probe_test();
   class_api_test();
   invalid_config_test();
   zoned_and_type_erased_test();
   cuckoo_test();
+  learned_config_test();
   std::printf("ALL PASS\n");
   return 0;
 }
