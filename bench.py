@@ -574,8 +574,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
-            "kernel": ("partitioned probe phase (k_part_hist, k_part_scatter, k_bbf_probe_global over the "
-                       "grouped keys, match reorder), HIP events around the phase" if partitioned
+            "kernel": ("partitioned probe phase (k_part_hist, k_part_scatter, k_bbf_probe_split over the "
+                       "grouped keys: 4 lanes per key, one 16 B piece of the 64 B block each; match "
+                       "reorder), HIP events around the phase" if partitioned
                        else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
             "bytes_per_probe": bytes_per_key,

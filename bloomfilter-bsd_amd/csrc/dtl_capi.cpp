@@ -258,6 +258,12 @@ int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, ui
 
 int dtl_bbf_config_count(void) { return static_cast<int>(bbf_registry().size()); }
 
+int dtl_bbf_split_lanes(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k) {
+  if (z > 1) s = w;
+  const BbfEntry* e = bbf_find(word_bits, w, s, z > 1 ? z : 1, k);
+  return e ? static_cast<int>(e->split_lanes) : 0;
+}
+
 int dtl_bbf_desc_init(dtl_bbf_desc* d, uint32_t word_bits, uint64_t m, uint32_t k, uint32_t w,
                       uint32_t s, uint32_t z) {
   if (!d) return DTL_EINVAL;
@@ -378,6 +384,11 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
   if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
   const bool use_lds = !(flags & DTL_PROBE_FORCE_GLOBAL) && d->size_bytes <= lds_max;
+  // Lane split pays where blocks come from L2 / Infinity Cache (1.33x at
+  // 4 MiB, and in the partitioned probe); a direct probe of a filter beyond
+  // the Infinity Cache is bound by HBM line fills, where one lane per key is
+  // ~1% faster (measured on MI355X).
+  int split = (flags & DTL_PROBE_LANE_SPLIT) ? 1 : (flags & DTL_PROBE_ONE_LANE) ? 0 : 1;
 
   // partitioned probe: slices of 2^slice_log2 bytes (at most kPartMax of them)
   uint32_t slice_shift = 0, nparts = 0;
@@ -435,6 +446,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     P.keys_p = reinterpret_cast<uint32_t*>(sp);
     P.pos_p = reinterpret_cast<uint32_t*>(sp + pair_bytes);
     P.ntiles = ntiles;
+    P.split = split;
     P.cu_count = di.cu_count;
     P.stream = st;
     if (params && params->ev_start &&
@@ -450,6 +462,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));
   }
 
+  if (!(flags & (DTL_PROBE_LANE_SPLIT | DTL_PROBE_ONE_LANE)) && d->size_bytes >= (256ull << 20)) split = 0;
   void* scratch = nullptr;
   rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
   if (rc != DTL_OK) return rc;
@@ -467,6 +480,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   if (threads == 0) threads = d->size_bytes > 40 * 1024 ? 1024 : 256;
   L.lds_threads = threads >= 1024 ? 1024 : 256;
   L.lds_grid = params ? params->grid_blocks : 0;
+  L.split = split;
   L.cu_count = di.cu_count;
   L.stream = st;
   if (params && params->ev_start &&

@@ -26,6 +26,7 @@ struct BbfProbeLaunch {
   int use_lds;
   uint32_t lds_threads;
   uint32_t lds_grid;      // 0 = from occupancy
+  int split;              // lane-split global kernel where the configuration has one
   int cu_count;
   hipStream_t stream;
 };
@@ -93,6 +94,7 @@ struct BbfPartLaunch {
   uint64_t* bits;         // match bitmap in key order, ntiles * kTileWords words
   uint32_t* tile_cnt;     // ntiles
   uint32_t ntiles;
+  int split;              // lane-split probe kernel where the configuration has one
   int cu_count;
   hipStream_t stream;
 };
@@ -106,6 +108,7 @@ struct BbfEntry {
   hipError_t (*build)(const BbfBuildLaunch&);
   hipError_t (*probe_batched)(const BbfBatchLaunch&);  // may be null
   hipError_t (*probe_part)(const BbfPartLaunch&);      // probe phase of the partitioned path
+  uint32_t split_lanes;                                // lanes per key of the lane-split kernel, 0 = none
 };
 
 constexpr uint32_t kPartMax = 2048;   // partitions of the partitioned probe

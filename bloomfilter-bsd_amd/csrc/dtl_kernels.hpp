@@ -223,6 +223,173 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
 }
 
 // ---------------------------------------------------------------------------
+// Lane-split probe (configurations with C::split.ok): L = block bytes / 16
+// lanes share a key, lane `piece` = lane % L loads the key's 16-byte piece
+// `piece` of the block and tests the groups that live there.  One 16-byte
+// load instruction of a wave then covers 64 / L whole blocks instead of a
+// quarter (64 B block) of 64 blocks, which is what bounds gathers from L2:
+// the L1/TA handles an instruction per distinct cache line it touches.
+// The key's hit is the AND of its L lanes' ballot bits (and_compress).
+// Same grid, tile order and outputs as k_bbf_probe_global.
+// ---------------------------------------------------------------------------
+#ifndef DTL_SPLIT_UNROLL
+#define DTL_SPLIT_UNROLL 16  // 16-byte loads in flight per lane
+#endif
+
+// Bit j of the result = AND of bits [j*L, (j+1)*L) of m (64 / L result bits).
+template <int L>
+__device__ __forceinline__ uint64_t and_compress(uint64_t m) {
+  if constexpr (L == 2) {
+    m &= m >> 1;
+    m &= 0x5555555555555555ull;
+    m = (m | (m >> 1)) & 0x3333333333333333ull;
+    m = (m | (m >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    m = (m | (m >> 4)) & 0x00FF00FF00FF00FFull;
+    m = (m | (m >> 8)) & 0x0000FFFF0000FFFFull;
+    m = (m | (m >> 16)) & 0x00000000FFFFFFFFull;
+  } else if constexpr (L == 4) {
+    m &= m >> 1;
+    m &= m >> 2;
+    m &= 0x1111111111111111ull;
+    m = (m | (m >> 3)) & 0x0303030303030303ull;
+    m = (m | (m >> 6)) & 0x000F000F000F000Full;
+    m = (m | (m >> 12)) & 0x000000FF000000FFull;
+    m = (m | (m >> 24)) & 0x000000000000FFFFull;
+  } else {
+    static_assert(L == 8, "2, 4 or 8 lanes per key");
+    m &= m >> 1;
+    m &= m >> 2;
+    m &= m >> 4;
+    m &= 0x0101010101010101ull;
+    m = (m | (m >> 7)) & 0x0003000300030003ull;
+    m = (m | (m >> 14)) & 0x0000000F0000000Full;
+    m = (m | (m >> 28)) & 0x00000000000000FFull;
+  }
+  return m;
+}
+
+// Per-lane folded multipliers of the lane's piece: fb for the bits, fd for
+// the dynamic word offsets (split_fold; piece selected at run time once).
+template <class C>
+struct SplitLane {
+  static constexpr SplitSched P = C::split;
+  static constexpr int NB = P.gpp * P.bpg;
+  uint32_t fb[NB];
+  uint32_t fd[P.gpp];
+  __device__ __forceinline__ explicit SplitLane(uint32_t piece) {
+    static_for<NB>([&](auto jj) {
+      constexpr int j = decltype(jj)::value, gl = j / P.bpg, i = j % P.bpg;
+      uint32_t v = split_fold(C::sched.bbit[gl * P.bpg + i]);
+      static_for<P.lanes - 1>([&](auto pp) {
+        constexpr int p = decltype(pp)::value + 1;
+        constexpr uint32_t f = split_fold(C::sched.bbit[(p * P.gpp + gl) * P.bpg + i]);
+        v = piece == static_cast<uint32_t>(p) ? f : v;
+      });
+      fb[j] = v;
+    });
+    static_for<P.gpp>([&](auto gg) {
+      constexpr int gl = decltype(gg)::value;
+      uint32_t v = 0;
+      if constexpr (P.dbits[gl] != 0) {
+        v = split_fold(C::sched.gdyn[gl]);
+        static_for<P.lanes - 1>([&](auto pp) {
+          constexpr int p = decltype(pp)::value + 1;
+          constexpr uint32_t f = split_fold(C::sched.gdyn[p * P.gpp + gl]);
+          v = piece == static_cast<uint32_t>(p) ? f : v;
+        });
+      }
+      fd[gl] = v;
+    });
+  }
+
+  // Test of the groups in this lane's piece of the key's block.
+  __device__ __forceinline__ bool test(uint32_t key, const uint4& piece_data) const {
+    using W = typename C::word_t;
+    W pw[P.wpp];
+    __builtin_memcpy(pw, &piece_data, 16);
+    bool ok = true;
+    static_for<P.gpp>([&](auto gg) {
+      constexpr int gl = decltype(gg)::value;
+      W mask = 0;
+      static_for<P.bpg>([&](auto ii) {
+        constexpr int j = gl * P.bpg + decltype(ii)::value;
+        const uint32_t bit = P.bbase[j] + ((key * fb[j]) >> (32 - P.bbits[j]));
+        mask |= one_at<W>(bit);
+      });
+      W word;
+      if constexpr (P.dbits[gl] == 0) {
+        word = pw[P.lword[gl]];
+      } else {
+        word = select_word<P.lword[gl], (1 << P.dbits[gl])>(pw, (key * fd[gl]) >> (32 - P.dbits[gl]));
+      }
+      ok &= (word & mask) == mask;
+    });
+    return ok;
+  }
+};
+
+template <class C>
+__global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_split(
+    const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
+    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
+    uint32_t tiles_per_xcd, uint32_t ntiles) {
+  using W = typename C::word_t;
+  constexpr SplitSched P = C::split;
+  static_assert(P.ok, "configuration has no lane-split form");
+  constexpr int L = P.lanes;
+  constexpr int KPI = 64 / L;                   // keys per wave load instruction
+  constexpr int BT = kGlobalThreads;
+  constexpr int ROUNDS = kTileKeys / BT;        // 64-key groups per wave and tile
+  constexpr int U = DTL_SPLIT_UNROLL < ROUNDS * L ? DTL_SPLIT_UNROLL : ROUNDS * L;
+  constexpr int RU = U / L;                     // 64-key groups per batch
+  static_assert(U % L == 0 && ROUNDS % RU == 0, "unroll must cover whole 64-key groups");
+  constexpr uint64_t BYTES = C::w * sizeof(W);
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tile =
+      tiles_per_xcd ? (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t piece = lane & (L - 1), kl = lane / L;
+  const SplitLane<C> sl(piece);
+  const char* fbase = reinterpret_cast<const char*>(filter) + piece * 16u;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r0 = 0; r0 < ROUNDS; r0 += RU) {
+    uint32_t kr[RU];
+#pragma unroll
+    for (int j = 0; j < RU; ++j) {
+      const uint64_t i = tile_base + static_cast<uint64_t>(r0 + j) * BT + tid;
+      kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
+    }
+    uint32_t key[U];
+    uint4 dat[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) key[u] = __shfl(kr[u / L], (u % L) * KPI + kl);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t b = block_of(key[u] * kMul.v[0], addr);
+      dat[u] = *reinterpret_cast<const uint4*>(fbase + static_cast<uint64_t>(b) * BYTES);
+    }
+#pragma unroll
+    for (int j = 0; j < RU; ++j) {
+      const uint64_t g0 = tile_base + static_cast<uint64_t>(r0 + j) * BT + wave * 64u;
+      uint64_t word = 0;
+#pragma unroll
+      for (int p = 0; p < L; ++p) {
+        const int u = j * L + p;
+        const bool ok = sl.test(key[u], dat[u]) & (g0 + p * KPI + kl < n);
+        word |= and_compress<L>(__ballot(ok)) << (p * KPI);
+      }
+      if (lane == 0) store_bits(bits, g0 / 64, word);
+      cnt += __popcll(word);
+    }
+  }
+  const uint32_t total = block_sum_waves<BT>(cnt, red);
+  if (tid == 0) tile_cnt[tile] = total;
+}
+
+// ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
 // ---------------------------------------------------------------------------
 template <class C, int BT>

@@ -53,20 +53,28 @@ hipError_t launch_probe(const BbfProbeLaunch& L) {
       default: return launch_probe_lds<C, 256>(L);
     }
   }
-  hipLaunchKernelGGL(k_bbf_probe_global<C>, dim3(L.ntiles), dim3(kGlobalThreads), 0, L.stream,
+  auto fn = k_bbf_probe_global<C>;
+  if constexpr (C::split.ok) {
+    if (L.split) fn = k_bbf_probe_split<C>;
+  }
+  hipLaunchKernelGGL(fn, dim3(L.ntiles), dim3(kGlobalThreads), 0, L.stream,
                      static_cast<const W*>(L.words), L.keys, L.n,
                      make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post),
                      L.bits, L.tile_cnt, 0u, L.ntiles);
   return hipGetLastError();
 }
 
-// Probe phase of the partitioned path: the global kernel over the grouped
-// keys, XCD-ordered, into the partitioned-order bitmap.
+// Probe phase of the partitioned path: the global (or lane-split) kernel
+// over the grouped keys, XCD-ordered, into the partitioned-order bitmap.
 template <class C>
 hipError_t launch_probe_part(const BbfPartLaunch& L) {
   using W = typename C::word_t;
   const uint32_t per_xcd = (L.ntiles + 7u) / 8u;
-  hipLaunchKernelGGL(k_bbf_probe_global<C>, dim3(per_xcd * 8u), dim3(kGlobalThreads), 0, L.stream,
+  auto fn = k_bbf_probe_global<C>;
+  if constexpr (C::split.ok) {
+    if (L.split) fn = k_bbf_probe_split<C>;
+  }
+  hipLaunchKernelGGL(fn, dim3(per_xcd * 8u), dim3(kGlobalThreads), 0, L.stream,
                      static_cast<const W*>(L.words), L.keys_p, L.n,
                      make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post), L.pbits,
                      L.ptile_cnt, per_xcd, L.ntiles);
@@ -97,6 +105,7 @@ BbfEntry make_entry() {
   e.build = &launch_build<C>;
   e.probe_batched = &launch_probe_batched<C>;
   e.probe_part = &launch_probe_part<C>;
+  e.split_lanes = C::split.ok ? static_cast<uint32_t>(C::split.lanes) : 0u;
   return e;
 }
 

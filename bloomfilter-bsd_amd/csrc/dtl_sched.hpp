@@ -168,6 +168,76 @@ constexpr Sched make_sched(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t 
   return S;
 }
 
+// Lane-split form of a schedule.  A block of 32 B or more is read by
+// `lanes` = block bytes / 16 lanes, one 16-byte piece each, and every lane
+// tests only the groups whose candidate words lie in its piece.  This needs
+// every group to stay inside one piece and every piece to hold the same
+// pattern of groups (same first word relative to the piece, same dynamic
+// word-offset width, same bits per group with the same widths and bases);
+// only the hash sources and shifts differ from piece to piece, and a lane
+// folds those into per-lane multipliers (split_fold) once per kernel.
+// Holds for s >= w (one group per word), for s < w with sectors of at most
+// 16 B and for zoned filters with zones of at most 16 B.
+struct SplitSched {
+  bool ok = false;
+  int lanes = 1;           // 16-byte pieces per block
+  int wpp = 0;             // words per piece
+  int gpp = 0;             // groups per piece
+  int bpg = 0;             // bits per group
+  uint8_t lword[16] = {};  // first word of local group gl, relative to its piece
+  uint8_t dbits[16] = {};  // dynamic word-offset width of local group gl
+  uint8_t bbits[16] = {};  // width of local bit gl * bpg + i
+  uint8_t bbase[16] = {};  // base of local bit gl * bpg + i
+};
+
+constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w) {
+  SplitSched P{};
+  const uint32_t bytes = w * word_bytes;
+  if (bytes < 32 || S.ngroups == 0) return P;
+  const int L = static_cast<int>(bytes / 16);
+  const int wpp = static_cast<int>(16 / word_bytes);
+  if (S.ngroups % L != 0 || S.nbits % S.ngroups != 0) return P;
+  const int gpp = S.ngroups / L, bpg = S.nbits / S.ngroups;
+  for (int b = 0; b < S.nbits; ++b)
+    if (S.bgroup[b] != b / bpg) return P;
+  for (int g = 0; g < S.ngroups; ++g) {
+    const int piece = g / gpp, gl = g % gpp;
+    const int first = S.gword[g], last = first + (1 << S.gdyn[g].nbits) - 1;
+    if (first / wpp != piece || last / wpp != piece) return P;
+    const int lw = first - piece * wpp;
+    if (piece == 0) {
+      P.lword[gl] = static_cast<uint8_t>(lw);
+      P.dbits[gl] = S.gdyn[g].nbits;
+    } else if (P.lword[gl] != lw || P.dbits[gl] != S.gdyn[g].nbits) {
+      return P;
+    }
+    for (int i = 0; i < bpg; ++i) {
+      const Chunk c = S.bbit[g * bpg + i];
+      const int j = gl * bpg + i;
+      if (c.nbits == 0) return P;
+      if (piece == 0) {
+        P.bbits[j] = c.nbits;
+        P.bbase[j] = c.base;
+      } else if (P.bbits[j] != c.nbits || P.bbase[j] != c.base) {
+        return P;
+      }
+    }
+  }
+  P.ok = true;
+  P.lanes = L;
+  P.wpp = wpp;
+  P.gpp = gpp;
+  P.bpg = bpg;
+  return P;
+}
+
+// ((H[src] >> shift) & ((1 << nbits) - 1)) == (key * split_fold(c)) >> (32 - nbits)
+// for nbits >= 1: shifting the multiplier left by 32 - nbits - shift moves the
+// chunk's bits of the product to its top (H[0] = 0 folds to multiplier 0).
+constexpr uint32_t split_fold(Chunk c) {
+  return c.src == 0 ? 0u : kMul.v[c.src] << (32u - c.nbits - c.shift);
+}
+
 // Validity of a configuration exactly as the reference's runtime dispatch
 // accepts it (blocked_bloomfilter.hpp:169-229; zoned_blocked_bloomfilter.hpp:102-215).
 constexpr bool valid_sgew_sltw(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t k) {
@@ -197,6 +267,7 @@ struct BbfCfg {
   static constexpr uint32_t z = Zc;
   static constexpr uint32_t k = Kc;
   static constexpr Sched sched = make_sched(word_bits, Wc, Sc, Zc, Kc);
+  static constexpr SplitSched split = make_split(sched, sizeof(W), Wc);
   static_assert(Zc > 1 ? valid_zoned(word_bits, Wc, Zc, Kc) : valid_sgew_sltw(word_bits, Wc, Sc, Kc),
                 "unsupported configuration");
 };

@@ -16,6 +16,8 @@ ADDR_POW2, ADDR_MAGIC = 0, 1
 PROBE_FORCE_GLOBAL = 1
 PROBE_PARTITIONED = 2
 PROBE_DIRECT = 4
+PROBE_LANE_SPLIT = 8
+PROBE_ONE_LANE = 16
 
 
 class BbfDesc(C.Structure):
@@ -78,7 +80,7 @@ EXPORTS = [
     "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
-    "dtl_bbf_config_count", "dtl_version", "dtl_gather_bench",
+    "dtl_bbf_config_count", "dtl_bbf_split_lanes", "dtl_version", "dtl_gather_bench",
     "bbf_batch_plan_group_count", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
     "dtl_model_fpr", "dtl_model_config_count", "dtl_model_get_config", "dtl_model_calibration_seconds",
     "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
@@ -120,6 +122,7 @@ def load(path=LIB_PATH):
     L.dtl_last_error.restype = C.c_char_p
     L.dtl_bbf_supported.argtypes = [u32, u32, u32, u32, u32]
     L.dtl_bbf_config_count.argtypes = []
+    L.dtl_bbf_split_lanes.argtypes = [u32, u32, u32, u32, u32]
     L.dtl_version.argtypes = []
     L.dtl_gather_bench.argtypes = [vp, u64, u32, u64, vp, vp]
     L.bbf_batch_plan_group_count.argtypes = [vp]
@@ -158,7 +161,8 @@ def load(path=LIB_PATH):
         getattr(L, name).restype = i32
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
                  "cf_probe", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
-                 "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_version",
+                 "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_bbf_split_lanes",
+                 "dtl_version",
                  "dtl_gather_bench"):
         getattr(L, name).restype = i32
     _lib = L

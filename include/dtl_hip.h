@@ -113,12 +113,16 @@ typedef struct dtl_probe_params {
 /* Probe strategy.  By default filters that fit in LDS are staged there; larger
  * filters of at least 64 MiB probed with at least 2^22 keys use the
  * partitioned probe (keys grouped by filter slice so that each slice is read
- * into L2 once); everything else reads blocks straight from memory.  The
- * result is the same list in every case. */
+ * into L2 once); everything else reads blocks straight from memory.  Blocks
+ * of 32 B or more whose bits stay inside 16-byte pieces (s >= w; sectors or
+ * zones of at most 16 B) are read by several lanes per key, one 16-byte piece
+ * each ("lane split").  The result is the same list in every case. */
 enum {
   DTL_PROBE_FORCE_GLOBAL = 1u,   /* never stage in LDS */
   DTL_PROBE_PARTITIONED = 2u,    /* always use the partitioned probe (filters larger than LDS) */
-  DTL_PROBE_DIRECT = 4u          /* never use the partitioned probe */
+  DTL_PROBE_DIRECT = 4u,         /* never use the partitioned probe */
+  DTL_PROBE_LANE_SPLIT = 8u,     /* always read blocks lane-split where the configuration allows */
+  DTL_PROBE_ONE_LANE = 16u       /* never: one lane loads a key's whole block */
 };
 
 /* ---- descriptors -------------------------------------------------------- */
@@ -269,6 +273,11 @@ const char* dtl_last_error(void);
 int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
 /* Number of compiled blocked Bloom filter configurations. */
 int dtl_bbf_config_count(void);
+/* Lanes per key (2, 4 or 8) of the lane-split probe kernel of a configuration
+ * (DTL_PROBE_LANE_SPLIT), 0 if it has none (blocks under 32 B, or bits whose
+ * word range crosses a 16-byte piece) or is not compiled.  No counterpart in
+ * the reference (its SIMD kernels have no such choice). */
+int dtl_bbf_split_lanes(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
 int dtl_version(void);
 
 #ifdef __cplusplus

@@ -76,6 +76,26 @@ def test_every_reference_configuration_is_compiled():
     assert not L.dtl_bbf_supported(32, 4, 4, 4, 8)
 
 
+def test_lane_split_classification():
+    """The lane-split kernel exists exactly where every word a bit can land
+    in lies inside one 16-byte piece of a block of at least 32 B: s >= w
+    (one sector per word or finer), sectors of at most 16 B (s < w), zones
+    of at most 16 B; lanes per key = block bytes / 16."""
+    L = D.load()
+    for wb, w, s, z, k in valid_configs():
+        block = w * wb // 8
+        if z > 1:
+            local = (w // z) * wb // 8 <= 16
+        else:
+            local = s >= w or (w // s) * wb // 8 <= 16
+        want = block // 16 if block >= 32 and local else 0
+        assert L.dtl_bbf_split_lanes(wb, w, s, z, k) == want, (wb, w, s, z, k)
+    assert L.dtl_bbf_split_lanes(64, 8, 8, 1, 8) == 4      # the cache-sectorized benchmark filter
+    assert L.dtl_bbf_split_lanes(64, 8, 2, 1, 8) == 0      # 32 B sectors cross pieces
+    assert L.dtl_bbf_split_lanes(32, 4, 4, 1, 4) == 0      # 16 B blocks: one lane already
+    assert L.dtl_bbf_split_lanes(32, 1, 8, 1, 8) == 0      # not a configuration
+
+
 @pytest.mark.parametrize("cfg", [
     (32, 1024, 8, 1, 1, 1), (32, 1 << 20, 4, 1, 1, 1), (64, 1 << 17, 8, 8, 8, 1),
     (64, 1 << 33, 8, 8, 8, 1), (32, 10 * 1000, 6, 16, 16, 2), (32, 10 * (1 << 22), 6, 16, 16, 2),

@@ -104,9 +104,7 @@ def test_lds_and_global_paths_agree(cfg):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-def test_every_configuration_parity():
-    """Every configuration the reference dispatches, POW2 and MAGIC sizes,
-    including the k=8 word-block configurations (scalar semantics)."""
+def all_configs():
     cfgs = []
     for wb in (32, 64):
         for w in (1, 2, 4, 8, 16):
@@ -122,10 +120,66 @@ def test_every_configuration_parity():
                     for k in range(2, 17, 2):
                         if k % z == 0:
                             cfgs.append((wb, w, w, z, k))
+    return cfgs
+
+
+def test_every_configuration_parity():
+    """Every configuration the reference dispatches, POW2 and MAGIC sizes,
+    including the k=8 word-block configurations (scalar semantics)."""
+    cfgs = all_configs()
     assert len(cfgs) == D.load().dtl_bbf_config_count()
     for i, (wb, w, s, z, k) in enumerate(cfgs):
         m = (1 << 16) if i % 2 == 0 else 50021
         check_cfg((wb, m, k, w, s, z), 800, 6000, seed=100 + i, hit=0.3, offset=i)
+
+
+def test_every_configuration_global_paths():
+    """Every configuration through the global-memory kernels (filter larger
+    than the LDS limit given): the lane-split kernel where the configuration
+    has one and the one-lane kernel, POW2 and MAGIC sizes, ragged last tile."""
+    L = D.load()
+    split_default = D.ProbeParams(1024, 0, 0, 0, D.PROBE_DIRECT)
+    one_lane = D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_DIRECT | D.PROBE_ONE_LANE)
+    n_split = 0
+    for i, (wb, w, s, z, k) in enumerate(all_configs()):
+        m = (1 << 18) if i % 2 == 0 else 200003
+        cfg = (wb, m, k, w, s, z)
+        f = O.Bloom(*cfg)
+        desc = D.bbf_desc(*cfg)
+        rng = np.random.default_rng(500 + i)
+        build = rng.integers(0, 2**31, 3000, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        ow = oracle_words(f, build)
+        assert np.array_equal(words.cpu().numpy().view(ow.dtype), ow), cfg
+        probe = mixed_keys(build, 5000, 0.3, 600 + i)
+        exp = f.batch_contains(ow, probe, i)
+        keys = to_dev(probe)
+        for params in (split_default, one_lane):
+            out, cnt = D.bbf_probe(desc, words, keys, i, params=params)
+            torch.cuda.synchronize()
+            assert np.array_equal(D.positions(out, cnt), exp), (cfg, params.flags)
+        n_split += L.dtl_bbf_split_lanes(wb, w, s, z, k) > 0
+    assert n_split == 79  # the configurations with a lane-split kernel (host test pins the rule)
+
+
+@pytest.mark.parametrize("cfg", [
+    (64, 1 << 25, 8, 8, 8, 1),        # 4 lanes per key, POW2
+    (64, 10 * 300001, 16, 16, 16, 1),  # 128 B blocks: 8 lanes per key, MAGIC
+    (32, 1 << 24, 8, 8, 8, 1),        # 32 B blocks: 2 lanes per key
+    (64, 1 << 24, 8, 8, 4, 1),        # sectors of 16 B, dynamic word inside the piece
+    (32, 10 * 400000, 12, 16, 16, 4),  # zones of 16 B
+    (64, 1 << 24, 6, 4, 4, 2),        # u64 zones of 16 B
+])
+def test_lane_split_partitioned_and_direct(cfg):
+    """The lane-split kernel over grouped keys (partitioned probe) and over
+    the keys in order, against the oracle and the one-lane kernel."""
+    assert D.load().dtl_bbf_split_lanes(cfg[0], cfg[3], cfg[4], cfg[5], cfg[2]) > 0
+    a = check_cfg(cfg, 60000, 300001, seed=41, hit=0.1, offset=9, params=part_params(14))
+    b = check_cfg(cfg, 60000, 300001, seed=41, hit=0.1, offset=9,
+                  params=D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_DIRECT))
+    c = check_cfg(cfg, 60000, 300001, seed=41, hit=0.1, offset=9,
+                  params=D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_DIRECT | D.PROBE_ONE_LANE))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 2047, 2048, 2049, 4097, 100003])
