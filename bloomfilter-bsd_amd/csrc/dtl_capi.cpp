@@ -389,6 +389,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   // the Infinity Cache is bound by HBM line fills, where one lane per key is
   // ~1% faster (measured on MI355X).
   int split = (flags & DTL_PROBE_LANE_SPLIT) ? 1 : (flags & DTL_PROBE_ONE_LANE) ? 0 : 1;
+  if (d->size_bytes > 0xFFFFFFFFull) split = 0;  // the lane-split kernel addresses with 32-bit offsets
 
   // partitioned probe: slices of 2^slice_log2 bytes (at most kPartMax of them)
   uint32_t slice_shift = 0, nparts = 0;

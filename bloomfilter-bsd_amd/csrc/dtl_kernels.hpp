@@ -22,6 +22,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "dtl_internal.hpp"
 #include "dtl_sched.hpp"
@@ -351,39 +352,53 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_split(
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t piece = lane & (L - 1), kl = lane / L;
   const SplitLane<C> sl(piece);
-  const char* fbase = reinterpret_cast<const char*>(filter) + piece * 16u;
+  // 32-bit byte offsets from the uniform base (the host keeps filters of
+  // this kernel under 4 GiB), so a load is one address op + saddr form.
+  const char* fbase = reinterpret_cast<const char*>(filter);
+  const uint32_t poff = piece * 16u;
   const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
   uint32_t cnt = 0;
+  // The loop is instantiated twice, for POW2 (block = h >> post) and MAGIC
+  // addressing; the branch is uniform over the grid.
+  auto run = [&](auto pow2) {
 #pragma unroll
-  for (int r0 = 0; r0 < ROUNDS; r0 += RU) {
-    uint32_t kr[RU];
+    for (int r0 = 0; r0 < ROUNDS; r0 += RU) {
+      uint32_t kr[RU];
 #pragma unroll
-    for (int j = 0; j < RU; ++j) {
-      const uint64_t i = tile_base + static_cast<uint64_t>(r0 + j) * BT + tid;
-      kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
-    }
-    uint32_t key[U];
-    uint4 dat[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) key[u] = __shfl(kr[u / L], (u % L) * KPI + kl);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t b = block_of(key[u] * kMul.v[0], addr);
-      dat[u] = *reinterpret_cast<const uint4*>(fbase + static_cast<uint64_t>(b) * BYTES);
-    }
-#pragma unroll
-    for (int j = 0; j < RU; ++j) {
-      const uint64_t g0 = tile_base + static_cast<uint64_t>(r0 + j) * BT + wave * 64u;
-      uint64_t word = 0;
-#pragma unroll
-      for (int p = 0; p < L; ++p) {
-        const int u = j * L + p;
-        const bool ok = sl.test(key[u], dat[u]) & (g0 + p * KPI + kl < n);
-        word |= and_compress<L>(__ballot(ok)) << (p * KPI);
+      for (int j = 0; j < RU; ++j) {
+        const uint64_t i = tile_base + static_cast<uint64_t>(r0 + j) * BT + tid;
+        kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
       }
-      if (lane == 0) store_bits(bits, g0 / 64, word);
-      cnt += __popcll(word);
+      uint32_t key[U];
+      uint4 dat[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) key[u] = __shfl(kr[u / L], (u % L) * KPI + kl);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t h = key[u] * kMul.v[0];
+        const uint32_t b = decltype(pow2)::value ? h >> addr.post : block_of(h, addr);
+        dat[u] = *reinterpret_cast<const uint4*>(fbase + (b * static_cast<uint32_t>(BYTES) + poff));
+      }
+#pragma unroll
+      for (int j = 0; j < RU; ++j) {
+        const uint64_t g0 = tile_base + static_cast<uint64_t>(r0 + j) * BT + wave * 64u;
+        uint64_t word = 0;
+#pragma unroll
+        for (int p = 0; p < L; ++p) {
+          const int u = j * L + p;
+          word |= and_compress<L>(__ballot(sl.test(key[u], dat[u]))) << (p * KPI);
+        }
+        // keys past n (last tile only): clear their bits, uniformly per wave
+        if (g0 + 64 > n) word &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
+        if (lane == 0) store_bits(bits, g0 / 64, word);
+        cnt += __popcll(word);
+      }
     }
+  };
+  if (addr.magic == 0) {
+    run(std::true_type{});
+  } else {
+    run(std::false_type{});
   }
   const uint32_t total = block_sum_waves<BT>(cnt, red);
   if (tid == 0) tile_cnt[tile] = total;

@@ -121,7 +121,8 @@ enum {
   DTL_PROBE_FORCE_GLOBAL = 1u,   /* never stage in LDS */
   DTL_PROBE_PARTITIONED = 2u,    /* always use the partitioned probe (filters larger than LDS) */
   DTL_PROBE_DIRECT = 4u,         /* never use the partitioned probe */
-  DTL_PROBE_LANE_SPLIT = 8u,     /* always read blocks lane-split where the configuration allows */
+  DTL_PROBE_LANE_SPLIT = 8u,     /* always read blocks lane-split where the configuration allows
+                                    (and the filter is under 4 GiB) */
   DTL_PROBE_ONE_LANE = 16u       /* never: one lane loads a key's whole block */
 };
 
