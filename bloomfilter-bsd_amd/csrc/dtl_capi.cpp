@@ -396,9 +396,11 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   if (!use_lds && !(flags & DTL_PROBE_DIRECT) && e->probe_part) {
     const bool want = (flags & DTL_PROBE_PARTITIONED) ||
                       (d->size_bytes >= (64ull << 20) && n >= (1ull << 22));
-    // slices of 2 MiB, 4 MiB from 512 MiB on (measured on MI355X)
+    // slices of 512 KiB, 1 MiB from 128 MiB on, 2 MiB from 512 MiB on
+    // (measured on MI355X with 2^26..2^28 probe keys)
     const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
-                                : d->size_bytes >= (512ull << 20) ? 22u : 21u;
+                                : d->size_bytes >= (512ull << 20) ? 21u
+                                : d->size_bytes >= (128ull << 20) ? 20u : 19u;
     const uint32_t bb_log2 = floor_log2(block_bytes(d));
     slice_shift = slice_log2 > bb_log2 ? slice_log2 - bb_log2 : 0u;
     while (slice_shift < 31 && ((static_cast<uint64_t>(d->block_cnt) + (1ull << slice_shift) - 1) >> slice_shift) > kPartMax)

@@ -144,8 +144,9 @@ __device__ __forceinline__ uint32_t load_key(const uint32_t* __restrict__ keys, 
   return __builtin_nontemporal_load(keys + i);
 }
 
+// Ordinary store: the bitmap is read back by the compaction right after.
 __device__ __forceinline__ void store_bits(uint64_t* __restrict__ bits, uint64_t i, uint64_t v) {
-  __builtin_nontemporal_store(v, bits + i);
+  bits[i] = v;
 }
 
 // Sums a per-wave uniform value over the workgroup; result valid in thread 0.

@@ -4,8 +4,8 @@
 // A probe of a large filter reads one random block per key; on gfx950 every
 // such read fills a whole 128-byte L2 line from HBM, so the direct kernel is
 // bound by random-line throughput.  The partitioned probe first groups the
-// keys by filter slice (slice = 2^slice_shift consecutive blocks, 2 MiB by
-// default), then probes slice after slice, so that a slice is read from HBM
+// keys by filter slice (slice = 2^slice_shift consecutive blocks, 512 KiB to
+// 2 MiB by default), then probes slice after slice, so that a slice is read from HBM
 // into one XCD's L2 once and every key of the partition hits it there.
 // Only key streams touch HBM: keys are read twice (histogram, scatter), the
 // (key, position) pairs written and read once.
@@ -16,9 +16,10 @@
 //   4. k_part_scatter   per tile of 16 keys per thread: counting sort in LDS, one global
 //                       atomicAdd per non-empty bin reserves the tile's run in
 //                       each partition, runs written out coalesced;
-//   5. k_bbf_probe_global (per configuration, dtl_kernels.hpp) probes the
-//                       grouped keys in XCD order and writes the match bitmap
-//                       in grouped order (ballot words, no atomics);
+//   5. k_bbf_probe_split / k_bbf_probe_global (per configuration,
+//                       dtl_kernels.hpp) probes the grouped keys in XCD order
+//                       and writes the match bitmap in grouped order (ballot
+//                       words, no atomics);
 //   6. reordering of the matches (only matches move, about 1-2% of keys):
 //      scan + k_expand_gather list the original positions of the matches,
 //      k_hit_hist / k_part_scan / k_hit_scatter bucket them by position
@@ -40,6 +41,7 @@ namespace dtlh {
 constexpr int kHistThreads = 256;
 constexpr int kHistUnroll = 4;
 constexpr int kScatterKpt = 16;  // keys per thread of the scatter (tile = threads * 16 keys)
+
 constexpr int kPartScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t part_of(uint32_t key, const AddrArgs& a, uint32_t slice_shift) {
@@ -65,7 +67,9 @@ __device__ __forceinline__ uint32_t lane_of(uint64_t i, uint32_t tile_log2) {
 }
 
 // Each workgroup counts keys [blockIdx * per_wg, +per_wg) (per_wg is a
-// multiple of kHistThreads * kHistUnroll) into nparts * kLanes bins.
+// multiple of kHistThreads * kHistUnroll) into nparts * kLanes bins.  In LDS
+// the bins are lane-major (lane * nparts + partition): the keys of one wave
+// share a lane, so their atomics spread over all banks.
 __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                             AddrArgs a, uint32_t slice_shift,
                                                             uint32_t nparts, uint32_t per_wg,
@@ -90,12 +94,14 @@ __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __re
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
       const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
-      if (valid[u]) atomicAdd(&hist[part_of(key[u], a, slice_shift) * kLanes + lane_of(i, tile_log2)], 1u);
+      if (valid[u]) atomicAdd(&hist[lane_of(i, tile_log2) * nparts + part_of(key[u], a, slice_shift)], 1u);
     }
   }
   __syncthreads();
-  for (uint32_t i = tid; i < nbins; i += kHistThreads)
-    if (hist[i]) atomicAdd(&counts[i], hist[i]);
+  for (uint32_t i = tid; i < nbins; i += kHistThreads) {
+    const uint32_t c = hist[(i % kLanes) * nparts + i / kLanes];  // global bin i = partition * kLanes + lane
+    if (c) atomicAdd(&counts[i], c);
+  }
 }
 
 // One workgroup: offsets[p] = sum of counts[< p], offsets[nparts] = total;
@@ -120,10 +126,17 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
   if (threadIdx.x == 0) offsets[nparts] = carry;
 }
 
-// LDS layout (dynamic): hist | lbase | gbase (nparts u32 each) | skey (tile
-// u32) | sidx (tile u16).
+// LDS layout (dynamic): hist | lbase | gbase (nparts u32 each) | buf (tile
+// u32).  The runs are written with ordinary (cached) stores: each tile's run
+// in a partition is ~64 keys and shares its first and last 128-byte lines
+// with the neighbouring runs of the same lane, which L2 merges; non-temporal
+// stores of the same runs made the scatter 1.7x slower on MI355X.  The tile goes out in two passes through the same buffer, keys first,
+// then positions, so a workgroup needs ~4 bytes of LDS per key and two
+// workgroups fit on a CU: one loads and sorts while the other writes (the
+// launch bound asks for 2 workgroups' waves per SIMD, <= 64 VGPRs at 1024
+// threads).
 template <int kScatterThreads>
-__global__ __launch_bounds__(kScatterThreads) void k_part_scatter(const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(kScatterThreads, kScatterThreads / 128) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, AddrArgs a,
                                                                   uint32_t slice_shift, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
@@ -135,23 +148,24 @@ __global__ __launch_bounds__(kScatterThreads) void k_part_scatter(const uint32_t
   uint32_t* hist = sm;
   uint32_t* lbase = sm + nparts;
   uint32_t* gbase = sm + 2 * nparts;
-  uint32_t* skey = sm + 3 * nparts;
-  uint16_t* sidx = reinterpret_cast<uint16_t*>(skey + kScatterTile);
+  uint32_t* buf = sm + 3 * nparts;
   const uint32_t tid = threadIdx.x;
   const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kScatterTile;
+  const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
   for (uint32_t i = tid; i < nparts; i += kScatterThreads) hist[i] = 0u;
   __syncthreads();
-  uint32_t key[kScatterKpt], pid[kScatterKpt], rank[kScatterKpt];
+  uint32_t key[kScatterKpt], slot[kScatterKpt];
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
-    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
-    key[r] = load_key(keys, i < n ? i : n - 1);
+    const uint32_t j = r * kScatterThreads + tid;
+    key[r] = load_key(keys, tile_base + (j < cnt ? j : cnt - 1));
   }
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
-    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
-    pid[r] = part_of(key[r], a, slice_shift);
-    rank[r] = i < n ? atomicAdd(&hist[pid[r]], 1u) : 0u;
+    const uint32_t j = r * kScatterThreads + tid;
+    const uint32_t pid = part_of(key[r], a, slice_shift);
+    // rank in the partition (low 16 bits, tile < 2^16) and partition id
+    slot[r] = j < cnt ? (atomicAdd(&hist[pid], 1u) | (pid << 16)) : 0u;
   }
   __syncthreads();
   // exclusive scan of the histogram (each thread owns `per` consecutive bins)
@@ -169,23 +183,37 @@ __global__ __launch_bounds__(kScatterThreads) void k_part_scatter(const uint32_t
     run += c;
   }
   __syncthreads();
+  uint32_t ps[kScatterKpt / 2];  // LDS slots (< 2^16), two per register
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
-    const uint64_t i = tile_base + static_cast<uint64_t>(r) * kScatterThreads + tid;
-    if (i < n) {
-      const uint32_t slot = lbase[pid[r]] + rank[r];
-      skey[slot] = key[r];
-      sidx[slot] = static_cast<uint16_t>(r * kScatterThreads + tid);
+    const uint32_t j = r * kScatterThreads + tid;
+    const uint32_t sv = lbase[slot[r] >> 16] + (slot[r] & 0xFFFFu);
+    if (j < cnt) buf[sv] = key[r];
+    ps[r / 2] = (r & 1) ? (ps[r / 2] | (sv << 16)) : sv;
+  }
+  __syncthreads();
+  uint32_t dst[kScatterKpt];
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint32_t sl = r * kScatterThreads + tid;
+    if (sl < cnt) {
+      const uint32_t k = buf[sl];
+      const uint32_t p = part_of(k, a, slice_shift);
+      dst[r] = gbase[p] + (sl - lbase[p]);
+      keys_p[dst[r]] = k;
     }
   }
   __syncthreads();
-  const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
-  for (uint32_t slot = tid; slot < cnt; slot += kScatterThreads) {
-    const uint32_t k = skey[slot];
-    const uint32_t p = part_of(k, a, slice_shift);
-    const uint32_t dst = gbase[p] + (slot - lbase[p]);
-    __builtin_nontemporal_store(k, keys_p + dst);
-    __builtin_nontemporal_store(static_cast<uint32_t>(tile_base) + sidx[slot], pos_p + dst);
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint32_t j = r * kScatterThreads + tid;
+    if (j < cnt) buf[(ps[r / 2] >> (16 * (r & 1))) & 0xFFFFu] = static_cast<uint32_t>(tile_base) + j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScatterKpt; ++r) {
+    const uint32_t sl = r * kScatterThreads + tid;
+    if (sl < cnt) pos_p[dst[r]] = buf[sl];
   }
 }
 
@@ -244,7 +272,7 @@ static uint32_t fill_grid(uint64_t count) {
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((count + 255) / 256, 8192)));
 }
 
-static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 6u; }
+static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 4u; }
 
 template <int BT>
 static hipError_t launch_scatter(const BbfPartLaunch& L, const AddrArgs& a) {

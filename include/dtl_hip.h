@@ -106,7 +106,7 @@ typedef struct dtl_probe_params {
   void* ev_start;           /* optional hipEvent_t recorded right before the probe phase */
   void* ev_stop;            /* optional hipEvent_t recorded right after it (before compaction) */
   uint32_t slice_log2;      /* partitioned probe: log2 of the filter bytes per partition
-                               (default 21, 22 for filters of 512 MiB or more) */
+                               (default 19; 20 from 128 MiB, 21 from 512 MiB) */
   uint32_t reserved;
 } dtl_probe_params;
 
