@@ -234,8 +234,11 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
 // The key's hit is the AND of its L lanes' ballot bits (and_compress).
 // Same grid, tile order and outputs as k_bbf_probe_global.
 // ---------------------------------------------------------------------------
+#ifndef DTL_SPLIT_PIPE
+#define DTL_SPLIT_PIPE 0  // 1: double-buffered batches (next batch in flight while one is tested)
+#endif
 #ifndef DTL_SPLIT_UNROLL
-#define DTL_SPLIT_UNROLL 16  // 16-byte loads in flight per lane
+#define DTL_SPLIT_UNROLL (DTL_SPLIT_PIPE ? 8 : 16)  // 16-byte loads per lane per batch
 #endif
 
 // Bit j of the result = AND of bits [j*L, (j+1)*L) of m (64 / L result bits).
@@ -362,39 +365,57 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_split(
   // The loop is instantiated twice, for POW2 (block = h >> post) and MAGIC
   // addressing; the branch is uniform over the grid.
   auto run = [&](auto pow2) {
+    // every key word of the tile is requested up front (one per round)
+    uint32_t kr[ROUNDS];
 #pragma unroll
-    for (int r0 = 0; r0 < ROUNDS; r0 += RU) {
-      uint32_t kr[RU];
+    for (int j = 0; j < ROUNDS; ++j) {
+      const uint64_t i = tile_base + static_cast<uint64_t>(j) * BT + tid;
+      kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
+    }
+    // sub-round s covers keys (s % L) * KPI .. + KPI of round s / L
+    auto issue = [&](int s, uint32_t& key, uint4& dat) {
+      key = __shfl(kr[s / L], (s % L) * KPI + kl);
+      const uint32_t h = key * kMul.v[0];
+      const uint32_t b = decltype(pow2)::value ? h >> addr.post : block_of(h, addr);
+      dat = *reinterpret_cast<const uint4*>(fbase + (b * static_cast<uint32_t>(BYTES) + poff));
+    };
+    auto finish = [&](int round, const uint32_t (&key)[U], const uint4 (&dat)[U], int u0) {
+      const uint64_t g0 = tile_base + static_cast<uint64_t>(round) * BT + wave * 64u;
+      uint64_t word = 0;
 #pragma unroll
-      for (int j = 0; j < RU; ++j) {
-        const uint64_t i = tile_base + static_cast<uint64_t>(r0 + j) * BT + tid;
-        kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
+      for (int p = 0; p < L; ++p) word |= and_compress<L>(__ballot(sl.test(key[u0 + p], dat[u0 + p]))) << (p * KPI);
+      // keys past n (last tile only): clear their bits, uniformly per wave
+      if (g0 + 64 > n) word &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
+      if (lane == 0) store_bits(bits, g0 / 64, word);
+      cnt += __popcll(word);
+    };
+    constexpr int NB = ROUNDS / RU;  // batches of U sub-rounds
+#if DTL_SPLIT_PIPE
+    // double-buffered: batch b + 1 is in flight while batch b is tested
+    uint32_t key[2][U];
+    uint4 dat[2][U];
+#pragma unroll
+    for (int b = 0; b <= NB; ++b) {
+      if (b < NB) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) issue(b * U + u, key[b & 1][u], dat[b & 1][u]);
       }
+      if (b > 0) {
+#pragma unroll
+        for (int j = 0; j < RU; ++j) finish((b - 1) * RU + j, key[(b - 1) & 1], dat[(b - 1) & 1], j * L);
+      }
+    }
+#else
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
       uint32_t key[U];
       uint4 dat[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) key[u] = __shfl(kr[u / L], (u % L) * KPI + kl);
+      for (int u = 0; u < U; ++u) issue(b * U + u, key[u], dat[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t h = key[u] * kMul.v[0];
-        const uint32_t b = decltype(pow2)::value ? h >> addr.post : block_of(h, addr);
-        dat[u] = *reinterpret_cast<const uint4*>(fbase + (b * static_cast<uint32_t>(BYTES) + poff));
-      }
-#pragma unroll
-      for (int j = 0; j < RU; ++j) {
-        const uint64_t g0 = tile_base + static_cast<uint64_t>(r0 + j) * BT + wave * 64u;
-        uint64_t word = 0;
-#pragma unroll
-        for (int p = 0; p < L; ++p) {
-          const int u = j * L + p;
-          word |= and_compress<L>(__ballot(sl.test(key[u], dat[u]))) << (p * KPI);
-        }
-        // keys past n (last tile only): clear their bits, uniformly per wave
-        if (g0 + 64 > n) word &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
-        if (lane == 0) store_bits(bits, g0 / 64, word);
-        cnt += __popcll(word);
-      }
+      for (int j = 0; j < RU; ++j) finish(b * RU + j, key, dat, j * L);
     }
+#endif
   };
   if (addr.magic == 0) {
     run(std::true_type{});
