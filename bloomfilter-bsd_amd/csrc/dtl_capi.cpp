@@ -409,15 +409,20 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     if (!want || nparts < 2) nparts = 0;
   }
   if (nparts) {
-    const uint32_t nbuckets = (ntiles + kBucketTiles - 1) / kBucketTiles;
+    const uint32_t bucket_shift = hit_bucket_shift(ntiles);
+    const uint32_t bucket_tiles = (1u << bucket_shift) / kTileKeys;
+    const uint32_t nbuckets = (ntiles + bucket_tiles - 1) / bucket_tiles;
+    const uint32_t hit_groups = std::min<uint32_t>(kHitGroupsMax, 2u * static_cast<uint32_t>(std::max(di.cu_count, 1)));
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
     const size_t part_bytes = up((nbins * 3 + 1) * 4);
     const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
+    const size_t matrix_bytes = up(static_cast<size_t>(hit_groups) * nbuckets * 4);
     const size_t pair_bytes = up(static_cast<size_t>(n) * 4);
     void* scratch = nullptr;
-    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + 2 * pair_bytes, &scratch);
+    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + matrix_bytes + 2 * pair_bytes,
+                     &scratch);
     if (rc != DTL_OK) return rc;
     char* sp = static_cast<char*>(scratch);
     BbfPartLaunch P;
@@ -441,11 +446,15 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     P.cursor = P.offsets + nbins + 1;
     sp += part_bytes;
     P.d_hits = reinterpret_cast<uint64_t*>(sp);
+    P.bucket_shift = bucket_shift;
     P.nbuckets = nbuckets;
+    P.hit_groups = hit_groups;
     P.bcounts = reinterpret_cast<uint32_t*>(sp + 8);
     P.boffsets = P.bcounts + nbuckets;
     P.bcursor = P.boffsets + nbuckets + 1;
     sp += bucket_bytes;
+    P.hcounts = reinterpret_cast<uint32_t*>(sp);
+    sp += matrix_bytes;
     P.keys_p = reinterpret_cast<uint32_t*>(sp);
     P.pos_p = reinterpret_cast<uint32_t*>(sp + pair_bytes);
     P.ntiles = ntiles;

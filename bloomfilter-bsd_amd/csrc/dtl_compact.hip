@@ -2,11 +2,11 @@
 //
 // Phase 2 (scan): one workgroup per task scans the task's tile counts in
 // place into exclusive offsets and writes the task's match count.
-// Phase 3 (expand): one workgroup per tile (8 tiles per workgroup for single
-// lists); 256 threads each own 8 keys (one
-// byte of the bitmap), a workgroup exclusive scan of the byte popcounts gives
-// every match its rank, and positions i + match_offset are written in key
-// order: exactly the ascending selection vector of the reference
+// Phase 3 (expand): one workgroup per group of up to 8 tiles; 256 threads
+// each own 8 keys (one byte of the bitmap) of every tile of the group, one
+// workgroup exclusive scan of the byte popcounts gives every match its rank,
+// and positions i + match_offset are written in key order: exactly the
+// ascending selection vector of the reference
 // (blocked_bloomfilter_batch_dispatch.hpp:47-67).
 #include <hip/hip_runtime.h>
 
@@ -125,36 +125,53 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tasks(uint32_t* cnt,
 constexpr int kExpandThreads = 256;
 static_assert(kExpandThreads * 8 == kTileKeys, "one bitmap byte per thread");
 
-__device__ __forceinline__ void expand_tile(const uint64_t* __restrict__ bits, uint32_t gtile,
-                                            uint32_t ltile, uint32_t offset, uint32_t match_offset,
-                                            uint32_t* __restrict__ pos, uint32_t* sh) {
-  const uint32_t t = threadIdx.x;
-  const uint64_t word = bits[static_cast<uint64_t>(gtile) * kTileWords + (t >> 3)];
-  uint32_t b8 = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
-  uint32_t tot;
-  uint32_t out = offset + block_excl_scan<kExpandThreads>(__popc(b8), tot, sh);
-  const uint32_t kb = ltile * kTileKeys + (t >> 3) * 64u + (t & 7u) * 8u;
-  while (b8) {
-    const uint32_t j = __ffs(b8) - 1u;
-    pos[out++] = kb + j + match_offset;
-    b8 &= b8 - 1u;
-  }
-}
+// Expands up to kExpandGroup consecutive tiles [t0, t0 + nt) at once.  Every
+// thread loads its bitmap byte of each tile first (all loads in flight
+// together), one workgroup scan ranks the matches of all tiles (the 8
+// per-tile byte popcounts packed 16 bits each into two 64-bit values; a tile
+// has at most 2048 matches, so fields never carry into each other), and
+// emit(tile, key index inside the tile, output slot) runs for every match in
+// key order within its tile.
+constexpr int kExpandGroup = 8;
 
-// As expand_tile, but writes src[i] (a gathered value) instead of position i.
-__device__ __forceinline__ void expand_tile_gather(const uint64_t* __restrict__ bits, uint32_t tile,
-                                                   uint32_t offset, const uint32_t* __restrict__ src,
-                                                   uint32_t* __restrict__ out, uint32_t* sh) {
+template <class Emit>
+__device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
+                                             const uint32_t* __restrict__ offs, uint64_t* sh, Emit&& emit) {
   const uint32_t t = threadIdx.x;
-  const uint64_t word = bits[static_cast<uint64_t>(tile) * kTileWords + (t >> 3)];
-  uint32_t b8 = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
-  uint32_t tot;
-  uint32_t o = offset + block_excl_scan<kExpandThreads>(__popc(b8), tot, sh);
-  const uint64_t kb = static_cast<uint64_t>(tile) * kTileKeys + (t >> 3) * 64u + (t & 7u) * 8u;
-  while (b8) {
-    const uint32_t j = __ffs(b8) - 1u;
-    out[o++] = src[kb + j];
-    b8 &= b8 - 1u;
+  uint32_t b8[kExpandGroup];
+  // Every tile slot loads (tiles past the group's end re-read its last tile
+  // and are masked): straight-line loads, no branches around them.  A form
+  // that skipped the loads of missing tiles behind branches gave wrong
+  // bytes for tiles 4..7 of full groups on gfx950 (ROCm 7.2, -O3); this
+  // form is exact (tests/test_gpu_parity.py::test_large_batches_exact).
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) {
+    const uint32_t rr = static_cast<uint32_t>(r) < nt ? static_cast<uint32_t>(r) : nt - 1u;
+    const uint64_t w = bits[static_cast<uint64_t>(t0 + rr) * kTileWords + (t >> 3)];
+    const uint64_t word = static_cast<uint32_t>(r) < nt ? w : 0ull;
+    b8[r] = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
+  }
+  uint64_t lo = 0, hi = 0;
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) {
+    const uint64_t c = static_cast<uint64_t>(__popc(b8[r])) << (16 * (r % 4));
+    if (r < 4) lo |= c; else hi |= c;
+  }
+  uint64_t tot;
+  const uint64_t elo = block_excl_scan64<kExpandThreads>(lo, tot, sh);
+  const uint64_t ehi = block_excl_scan64<kExpandThreads>(hi, tot, sh);
+  const uint32_t kb = (t >> 3) * 64u + (t & 7u) * 8u;
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) {
+    if (static_cast<uint32_t>(r) >= nt) break;
+    const uint64_t ex = r < 4 ? elo : ehi;
+    uint32_t o = offs[t0 + r] + static_cast<uint32_t>((ex >> (16 * (r % 4))) & 0xFFFFu);
+    uint32_t b = b8[r];
+    while (b) {
+      const uint32_t j = __ffs(b) - 1u;
+      emit(t0 + r, kb + j, o++);
+      b &= b - 1u;
+    }
   }
 }
 
@@ -164,14 +181,17 @@ static uint32_t expand_tiles(uint32_t ntiles) {
   return ntiles >= 65536 ? 8u : ntiles >= 16384 ? 4u : ntiles >= 4096 ? 2u : 1u;
 }
 
+// Writes src[i] (a gathered value) for every match i instead of i itself.
 __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ offs,
                                                                   uint32_t ntiles, uint32_t per_wg,
                                                                   const uint32_t* __restrict__ src,
                                                                   uint32_t* __restrict__ out) {
-  __shared__ uint32_t sh[kExpandThreads / 64];
-  const uint32_t t1 = min(ntiles, (blockIdx.x + 1) * per_wg);
-  for (uint32_t t = blockIdx.x * per_wg; t < t1; ++t) expand_tile_gather(bits, t, offs[t], src, out, sh);
+  __shared__ uint64_t sh[kExpandThreads / 64];
+  const uint32_t t0 = blockIdx.x * per_wg;
+  expand_group(bits, t0, min(per_wg, ntiles - t0), offs, sh, [&](uint32_t tile, uint32_t i, uint32_t o) {
+    out[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i];
+  });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
@@ -179,9 +199,11 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t
                                                                   uint32_t ntiles, uint32_t per_wg,
                                                                   uint32_t match_offset,
                                                                   uint32_t* __restrict__ pos) {
-  __shared__ uint32_t sh[kExpandThreads / 64];
-  const uint32_t t1 = min(ntiles, (blockIdx.x + 1) * per_wg);
-  for (uint32_t t = blockIdx.x * per_wg; t < t1; ++t) expand_tile(bits, t, t, offs[t], match_offset, pos, sh);
+  __shared__ uint64_t sh[kExpandThreads / 64];
+  const uint32_t t0 = blockIdx.x * per_wg;
+  expand_group(bits, t0, min(per_wg, ntiles - t0), offs, sh, [&](uint32_t tile, uint32_t i, uint32_t o) {
+    pos[o] = tile * kTileKeys + i + match_offset;
+  });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
@@ -189,12 +211,12 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t*
                                                                  const BatchTaskDev* __restrict__ tasks,
                                                                  const uint32_t* __restrict__ tile_task,
                                                                  uint32_t total_tiles, uint32_t per_wg) {
-  __shared__ uint32_t sh[kExpandThreads / 64];
-  const uint32_t g1 = min(total_tiles, (blockIdx.x + 1) * per_wg);
-  for (uint32_t g = blockIdx.x * per_wg; g < g1; ++g) {
+  __shared__ uint64_t sh[kExpandThreads / 64];
+  const uint32_t g0 = blockIdx.x * per_wg;
+  expand_group(bits, g0, min(per_wg, total_tiles - g0), offs, sh, [&](uint32_t g, uint32_t i, uint32_t o) {
     const BatchTaskDev& t = tasks[tile_task[g]];
-    expand_tile(bits, g, g - t.tile_begin, offs[g], t.match_offset, t.pos, sh);
-  }
+    t.pos[o] = (g - t.tile_begin) * kTileKeys + i + t.match_offset;
+  });
 }
 
 __global__ void k_zero_count(uint64_t* d_count) { *d_count = 0; }

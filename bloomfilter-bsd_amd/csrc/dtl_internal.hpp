@@ -87,7 +87,10 @@ struct BbfPartLaunch {
   uint64_t* pbits;        // match bitmap in partitioned order, ntiles * kTileWords words
   uint32_t* ptile_cnt;    // ntiles
   uint64_t* d_hits;       // device: number of matches
-  uint32_t nbuckets;      // position buckets of kBucketKeys
+  uint32_t bucket_shift;  // positions per bucket of the hit sort: 2^bucket_shift
+  uint32_t nbuckets;      // position buckets (<= kHitBucketsMax)
+  uint32_t hit_groups;    // workgroups of the hit sort (<= kHitGroupsMax)
+  uint32_t* hcounts;      // hit_groups * nbuckets: per-workgroup bucket counts, then offsets
   uint32_t* bcounts;      // nbuckets
   uint32_t* boffsets;     // nbuckets + 1
   uint32_t* bcursor;      // nbuckets
@@ -99,8 +102,16 @@ struct BbfPartLaunch {
   hipStream_t stream;
 };
 
-constexpr uint32_t kBucketShift = 16;                 // positions per bucket of the hit sort
-constexpr uint32_t kBucketTiles = (1u << kBucketShift) / kTileKeys;
+// Hit sort of the partitioned probe: matches are bucketed by position,
+// 2^16 positions per bucket or more, so that there are at most
+// kHitBucketsMax buckets (an LDS histogram of them per workgroup).
+constexpr uint32_t kHitBucketsMax = 8192;
+constexpr uint32_t kHitGroupsMax = 512;
+inline uint32_t hit_bucket_shift(uint32_t ntiles) {
+  uint32_t s = 16;
+  while ((static_cast<uint64_t>(ntiles) * kTileKeys + (1ull << s) - 1) >> s > kHitBucketsMax) ++s;
+  return s;
+}
 
 struct BbfEntry {
   uint32_t word_bits, w, s, z, k;

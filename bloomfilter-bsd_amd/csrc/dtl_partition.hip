@@ -22,9 +22,10 @@
 //                       words, no atomics);
 //   6. reordering of the matches (only matches move, about 1-2% of keys):
 //      scan + k_expand_gather list the original positions of the matches,
-//      k_hit_hist / k_part_scan / k_hit_scatter bucket them by position
-//      (2^16 positions per bucket), and k_hit_bits builds each bucket's piece
-//      of the key-order bitmap in LDS and writes it with its tile counts.
+//      k_hit_count / k_hit_colscan / k_part_scan / k_hit_place bucket them by
+//      position (2^16 positions per bucket or more), and k_hit_bits builds
+//      each bucket's piece of the key-order bitmap in LDS and writes it with
+//      its tile counts.
 // The order of keys inside a partition depends on atomic timing, but the
 // key-order bitmap does not, so the match list is the same ascending list as
 // the direct probe's.
@@ -217,54 +218,104 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / 128) void k_part
   }
 }
 
-// Matches (original positions, hp[0 .. *d_hits)) -> bucket histogram.
-__global__ __launch_bounds__(256) void k_hit_hist(const uint32_t* __restrict__ hp, const uint64_t* __restrict__ d_hits,
-                                                  uint32_t* __restrict__ bcounts) {
+// Hit sort: the matches' original positions hp[0 .. *d_hits) (any order)
+// are bucketed by position without global atomics.  Workgroup g of G owns
+// the slice [h * g / G, h * (g + 1) / G) of hp:
+//   k_hit_count    LDS histogram of its slice -> hcounts[g][bucket];
+//   k_hit_colscan  per bucket, exclusive scan over g in place, and the
+//                  bucket totals (then k_part_scan -> bucket offsets);
+//   k_hit_place    writes its positions to bucket offset + hcounts[g][b] +
+//                  LDS rank (order inside a bucket is irrelevant: k_hit_bits
+//                  turns each bucket into a bitmap).
+__device__ __forceinline__ void hit_slice(const uint64_t* d_hits, uint64_t* b0, uint64_t* b1) {
   const uint64_t h = *d_hits;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h; i += stride)
-    atomicAdd(&bcounts[hp[i] >> kBucketShift], 1u);
+  *b0 = h * blockIdx.x / gridDim.x;
+  *b1 = h * (blockIdx.x + 1) / gridDim.x;
 }
 
-__global__ __launch_bounds__(256) void k_hit_scatter(const uint32_t* __restrict__ hp,
-                                                     const uint64_t* __restrict__ d_hits,
-                                                     uint32_t* __restrict__ bcursor, uint32_t* __restrict__ out) {
-  const uint64_t h = *d_hits;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h; i += stride) {
+__global__ __launch_bounds__(256) void k_hit_count(const uint32_t* __restrict__ hp,
+                                                   const uint64_t* __restrict__ d_hits, uint32_t shift,
+                                                   uint32_t nbuckets, uint32_t* __restrict__ hcounts) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < nbuckets; b += 256) hist[b] = 0u;
+  __syncthreads();
+  uint64_t b0, b1;
+  hit_slice(d_hits, &b0, &b1);
+  for (uint64_t i = b0 + tid; i < b1; i += 256) atomicAdd(&hist[hp[i] >> shift], 1u);
+  __syncthreads();
+  uint32_t* row = hcounts + static_cast<uint64_t>(blockIdx.x) * nbuckets;
+  for (uint32_t b = tid; b < nbuckets; b += 256) row[b] = hist[b];
+}
+
+__global__ __launch_bounds__(256) void k_hit_colscan(uint32_t* __restrict__ hcounts, uint32_t groups,
+                                                     uint32_t nbuckets, uint32_t* __restrict__ totals) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbuckets) return;
+  uint32_t run = 0;
+#pragma unroll 8
+  for (uint32_t g = 0; g < groups; ++g) {
+    uint32_t* c = hcounts + static_cast<uint64_t>(g) * nbuckets + b;
+    const uint32_t v = *c;
+    *c = run;
+    run += v;
+  }
+  totals[b] = run;
+}
+
+__global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ hp,
+                                                   const uint64_t* __restrict__ d_hits, uint32_t shift,
+                                                   uint32_t nbuckets, const uint32_t* __restrict__ hcounts,
+                                                   const uint32_t* __restrict__ boffsets,
+                                                   uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t sm[];
+  uint32_t* rank = sm;
+  uint32_t* base = sm + nbuckets;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t* row = hcounts + static_cast<uint64_t>(blockIdx.x) * nbuckets;
+  for (uint32_t b = tid; b < nbuckets; b += 256) {
+    rank[b] = 0u;
+    base[b] = boffsets[b] + row[b];
+  }
+  __syncthreads();
+  uint64_t b0, b1;
+  hit_slice(d_hits, &b0, &b1);
+  for (uint64_t i = b0 + tid; i < b1; i += 256) {
     const uint32_t p = hp[i];
-    out[atomicAdd(&bcursor[p >> kBucketShift], 1u)] = p;
+    const uint32_t b = p >> shift;
+    out[base[b] + atomicAdd(&rank[b], 1u)] = p;
   }
 }
 
-// One workgroup per bucket of 2^kBucketShift positions: its matches set bits
-// of an LDS bitmap, which is written out whole (zeros included) together
-// with the match count of each of its kBucketTiles tiles.
-constexpr uint32_t kBucketWords32 = (1u << kBucketShift) / 32u;
-
+// One workgroup per bucket of 2^shift positions: its matches set bits of an
+// LDS bitmap (dynamic, 2^shift bits), which is written out whole (zeros
+// included) together with the match count of each of its tiles.
 __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ sorted,
-                                                  const uint32_t* __restrict__ boffsets, uint32_t ntiles,
-                                                  uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt) {
-  __shared__ uint32_t lbits[kBucketWords32];
+                                                  const uint32_t* __restrict__ boffsets, uint32_t shift,
+                                                  uint32_t ntiles, uint64_t* __restrict__ bits,
+                                                  uint32_t* __restrict__ tile_cnt) {
+  extern __shared__ uint32_t lbits[];
+  const uint32_t words32 = (1u << shift) / 32u;
+  const uint32_t btiles = (1u << shift) / kTileKeys;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
-  for (uint32_t i = tid; i < kBucketWords32; i += 256) lbits[i] = 0u;
+  for (uint32_t i = tid; i < words32; i += 256) lbits[i] = 0u;
   __syncthreads();
   const uint32_t beg = boffsets[b], end = boffsets[b + 1];
   for (uint32_t i = beg + tid; i < end; i += 256) {
-    const uint32_t p = sorted[i] & ((1u << kBucketShift) - 1u);
+    const uint32_t p = sorted[i] & ((1u << shift) - 1u);
     atomicOr(&lbits[p >> 5], 1u << (p & 31u));
   }
   __syncthreads();
-  const uint32_t tile0 = b * kBucketTiles;
-  const uint32_t tiles = std::min(kBucketTiles, ntiles - tile0);
+  const uint32_t tile0 = b * btiles;
+  const uint32_t tiles = std::min(btiles, ntiles - tile0);
   for (uint32_t w = tid; w < tiles * kTileWords; w += 256) {
     const uint64_t v = static_cast<uint64_t>(lbits[2 * w]) | (static_cast<uint64_t>(lbits[2 * w + 1]) << 32);
     bits[static_cast<uint64_t>(tile0) * kTileWords + w] = v;
   }
-  if (tid < tiles) {
+  for (uint32_t t = tid; t < tiles; t += 256) {
     uint32_t c = 0;
-    for (uint32_t j = 0; j < 2 * kTileWords; ++j) c += __popc(lbits[tid * 2 * kTileWords + j]);
-    tile_cnt[tile0 + tid] = c;
+    for (uint32_t j = 0; j < 2 * kTileWords; ++j) c += __popc(lbits[t * 2 * kTileWords + j]);
+    tile_cnt[tile0 + t] = c;
   }
 }
 
@@ -317,16 +368,20 @@ hipError_t part_reorder_matches(const BbfPartLaunch& L) {
   // hp (the matches' original positions) overwrites the grouped keys
   if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ntiles, L.pos_p, L.keys_p, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(L.nbuckets)), dim3(256), 0, st, L.bcounts,
-                     static_cast<uint64_t>(L.nbuckets));
-  const uint32_t grid = static_cast<uint32_t>(std::max(L.cu_count, 1)) * 4u;
-  hipLaunchKernelGGL(k_hit_hist, dim3(grid), dim3(256), 0, st, L.keys_p, L.d_hits, L.bcounts);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, L.nbuckets, L.boffsets,
+  const uint32_t G = L.hit_groups, nb = L.nbuckets;
+  e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_place), nb * 8u);
+  if (e == hipSuccess) e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_bits), (1u << L.bucket_shift) / 8u);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, L.keys_p, L.d_hits, L.bucket_shift, nb,
+                     L.hcounts);
+  hipLaunchKernelGGL(k_hit_colscan, dim3((nb + 255) / 256), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, nb, L.boffsets,
                      L.bcursor);
   // bucketed positions overwrite the grouped positions
-  hipLaunchKernelGGL(k_hit_scatter, dim3(grid), dim3(256), 0, st, L.keys_p, L.d_hits, L.bcursor, L.pos_p);
-  hipLaunchKernelGGL(k_hit_bits, dim3(L.nbuckets), dim3(256), 0, st, L.pos_p, L.boffsets, L.ntiles, L.bits,
-                     L.tile_cnt);
+  hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, L.keys_p, L.d_hits, L.bucket_shift, nb,
+                     L.hcounts, L.boffsets, L.pos_p);
+  hipLaunchKernelGGL(k_hit_bits, dim3(nb), dim3(256), (1u << L.bucket_shift) / 8u, st, L.pos_p, L.boffsets,
+                     L.bucket_shift, L.ntiles, L.bits, L.tile_cnt);
   return hipGetLastError();
 }
 

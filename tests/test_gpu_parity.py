@@ -372,6 +372,44 @@ def test_batched_graph_replay():
 # ---------------------------------------------------------------------------
 # full size (1 GiB cache-sectorized filter): properties + exact sample
 # ---------------------------------------------------------------------------
+def big_probe_keys(build, n, seed):
+    """n probe keys (1% of them build keys) without materialising a
+    per-key random stream larger than needed."""
+    rng = np.random.default_rng(seed)
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(seed)).astype(np.uint32) | np.uint32(1 << 31)
+    nh = n // 100
+    k[rng.integers(0, n, nh)] = build[rng.integers(0, len(build), nh)]
+    return k
+
+
+@pytest.mark.parametrize("ntiles,path", [(4096, "direct"), (16384, "direct"), (65536, "direct"),
+                                         (16384, "partitioned"), (65536, "partitioned")])
+def test_large_batches_exact(ntiles, path):
+    """Batches long enough for the multi-tile compaction groups (2, 4 and 8
+    tiles per workgroup) and for the hit sort at scale, against the oracle."""
+    cfg = (64, 1 << 25, 8, 8, 8, 1) if path == "direct" else (64, 1 << 29, 8, 8, 8, 1)
+    f = O.Bloom(*cfg)
+    desc = D.bbf_desc(*cfg)
+    rng = np.random.default_rng(ntiles)
+    build = rng.integers(0, 2**31, 300000, dtype=np.uint64).astype(np.uint32)
+    words = gpu_build(desc, build)
+    ow = oracle_words(f, build)
+    n = ntiles * 2048 - 37
+    probe = big_probe_keys(build, n, ntiles + 1)
+    flags = D.PROBE_DIRECT if path == "direct" else D.PROBE_PARTITIONED
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 11, params=D.ProbeParams(0, 0, 0, 0, flags))
+    torch.cuda.synchronize()
+    got = D.positions(out, cnt)
+    # checker: the oracle's threaded AVX-512 restatement of batch_contains
+    # (tests/test_cpu_baseline.py pins it to the scalar oracle); offset added here
+    import cpu_baseline as CB
+    threads = min(16, CB.physical_cores())
+    _, _, lists = CB.bbf_probe(f, ow, probe, threads)
+    exp = np.concatenate(lists).astype(np.uint64) + 11
+    assert len(got) == len(exp)
+    assert np.array_equal(got.astype(np.uint64), exp)
+
+
 def test_full_size_1gib_properties():
     cfg = (64, 1 << 33, 8, 8, 8, 1)
     desc = D.bbf_desc(*[cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]])
