@@ -41,7 +41,12 @@ namespace dtlh {
 
 constexpr int kHistThreads = 256;
 constexpr int kHistUnroll = 4;
-constexpr int kScatterKpt = 16;  // keys per thread of the scatter (tile = threads * 16 keys)
+#ifndef DTL_SCATTER_KPT
+#define DTL_SCATTER_KPT 16
+#endif
+constexpr int kScatterKpt = DTL_SCATTER_KPT;  // keys per thread of the scatter (tile = threads * kScatterKpt)
+constexpr uint32_t kScatterKptLog2 = kScatterKpt == 32 ? 5u : 4u;
+static_assert(kScatterKpt == 16 || kScatterKpt == 32, "16 or 32 keys per thread");
 
 constexpr int kPartScanThreads = 1024;
 
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
 // launch bound asks for 2 workgroups' waves per SIMD, <= 64 VGPRs at 1024
 // threads).
 template <int kScatterThreads>
-__global__ __launch_bounds__(kScatterThreads, kScatterThreads / 128) void k_part_scatter(const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt)) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, AddrArgs a,
                                                                   uint32_t slice_shift, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
@@ -248,19 +253,44 @@ __global__ __launch_bounds__(256) void k_hit_count(const uint32_t* __restrict__ 
   for (uint32_t b = tid; b < nbuckets; b += 256) row[b] = hist[b];
 }
 
+// Column scan of hcounts: a workgroup takes kColB buckets, each split into
+// kColS segments of consecutive workgroup rows (one thread per (bucket,
+// segment)), so every thread has its segment's loads in flight at once.
+constexpr uint32_t kColB = 16, kColS = 16;
+static_assert(kColB * kColS == 256, "one thread per (bucket, segment)");
+
 __global__ __launch_bounds__(256) void k_hit_colscan(uint32_t* __restrict__ hcounts, uint32_t groups,
                                                      uint32_t nbuckets, uint32_t* __restrict__ totals) {
-  const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= nbuckets) return;
-  uint32_t run = 0;
-#pragma unroll 8
-  for (uint32_t g = 0; g < groups; ++g) {
-    uint32_t* c = hcounts + static_cast<uint64_t>(g) * nbuckets + b;
-    const uint32_t v = *c;
-    *c = run;
-    run += v;
+  __shared__ uint32_t seg[kColS][kColB + 1];
+  const uint32_t bl = threadIdx.x % kColB, sg = threadIdx.x / kColB;
+  const uint32_t b = blockIdx.x * kColB + bl;
+  const uint32_t per = (groups + kColS - 1) / kColS;
+  const uint32_t g0 = min(groups, sg * per), g1 = min(groups, g0 + per);
+  uint32_t sum = 0;
+  if (b < nbuckets) {
+    for (uint32_t g = g0; g < g1; ++g) sum += hcounts[static_cast<uint64_t>(g) * nbuckets + b];
   }
-  totals[b] = run;
+  seg[sg][bl] = sum;
+  __syncthreads();
+  if (sg == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < kColS; ++k) {
+      const uint32_t v = seg[k][bl];
+      seg[k][bl] = run;
+      run += v;
+    }
+    if (b < nbuckets) totals[b] = run;
+  }
+  __syncthreads();
+  uint32_t run = seg[sg][bl];
+  if (b < nbuckets) {
+    for (uint32_t g = g0; g < g1; ++g) {
+      uint32_t* c = hcounts + static_cast<uint64_t>(g) * nbuckets + b;
+      const uint32_t v = *c;
+      *c = run;
+      run += v;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ hp,
@@ -350,7 +380,7 @@ hipError_t partition_keys(const BbfPartLaunch& L) {
   per_wg = std::max<uint64_t>(unit, (per_wg + unit - 1) / unit * unit);
   const uint32_t hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
   const uint32_t threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
-  const uint32_t tile_log2 = threads == 1024 ? 14u : threads == 512 ? 13u : 12u;
+  const uint32_t tile_log2 = (threads == 1024 ? 10u : threads == 512 ? 9u : 8u) + kScatterKptLog2;
   hipLaunchKernelGGL(k_part_hist, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, a,
                      L.slice_shift, L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.counts, nbins, L.offsets,
@@ -374,7 +404,7 @@ hipError_t part_reorder_matches(const BbfPartLaunch& L) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, L.keys_p, L.d_hits, L.bucket_shift, nb,
                      L.hcounts);
-  hipLaunchKernelGGL(k_hit_colscan, dim3((nb + 255) / 256), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts);
+  hipLaunchKernelGGL(k_hit_colscan, dim3((nb + kColB - 1) / kColB), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, nb, L.boffsets,
                      L.bcursor);
   // bucketed positions overwrite the grouped positions
