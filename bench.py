@@ -119,7 +119,8 @@ def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
 
 GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal",
                 2: "4 lanes / 64 B sector, 1x16 B each", 3: "as 2, non-temporal",
-                4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector"}
+                4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector",
+                6: "as 2, uniform random sectors", 7: "as 0, uniform random sectors"}
 
 
 def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
@@ -128,7 +129,7 @@ def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
     sink = torch.zeros(1, dtype=torch.int64, device="cuda")
     out = {}
     for mode in modes:
-        sector = 32 if mode >= 4 else 64
+        sector = 32 if mode in (4, 5) else 64
         for _ in range(2):
             D.gather_bench(buf, mode, n, sink)
         torch.cuda.synchronize()
@@ -511,10 +512,10 @@ def main():
         return
 
     if args.gather_only:
-        res = {"random_sector_gather_1GiB": gather_table(torch, D, range(6))}
+        res = {"random_sector_gather_1GiB": gather_table(torch, D, range(8))}
         for lb in (21, 22, 24, 27, 30):
             res["by_size_%dMiB" % (1 << (lb - 20))] = {
-                m: r["G_sectors_per_s"] for m, r in gather_table(torch, D, (0, 2, 4, 5), reps=3, log2_bytes=lb).items()}
+                m: r["G_sectors_per_s"] for m, r in gather_table(torch, D, (0, 2, 6, 7), reps=3, log2_bytes=lb).items()}
         print(json.dumps(res), flush=True)
         return
 

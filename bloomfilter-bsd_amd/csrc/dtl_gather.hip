@@ -10,6 +10,11 @@
 //   3  as 2 with non-temporal loads
 //   4  one lane per 32 B sector, two 16-byte loads
 //   5  two lanes per 32 B sector, one 16-byte load each
+//   6  as 2, sectors drawn by a full-avalanche hash of the read index
+//   7  as 0, sectors drawn by the same hash
+// Modes 0-5 pick sector (i * 0x9E3779B1) >> shift, a low-discrepancy
+// sequence that spreads consecutive reads evenly over the memory channels;
+// modes 6-7 pick independent uniform sectors, like the blocks of random keys.
 #include <hip/hip_runtime.h>
 
 #include "dtl_hip.h"
@@ -24,11 +29,23 @@ __device__ __forceinline__ uint32_t sector_of(uint64_t i, uint32_t shift) {
   return (static_cast<uint32_t>(i) * 0x9E3779B1u) >> shift;
 }
 
+// MurmurHash3's 32-bit finalizer (fmix32, Austin Appleby, public domain).
+__device__ __forceinline__ uint32_t sector_of_random(uint64_t i, uint32_t shift) {
+  uint32_t h = static_cast<uint32_t>(i);
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h >> shift;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, uint32_t shift,
                                                 uint64_t n_reads, uint64_t* __restrict__ sink) {
   constexpr bool NT = (MODE == 1 || MODE == 3);
-  constexpr int LANES = (MODE == 2 || MODE == 3) ? 4 : (MODE == 5 ? 2 : 1);
+  constexpr bool RANDOM = (MODE == 6 || MODE == 7);
+  constexpr int LANES = (MODE == 2 || MODE == 3 || MODE == 6) ? 4 : (MODE == 5 ? 2 : 1);
   constexpr int V = (MODE == 4) ? 2 : ((MODE == 5 || LANES == 4) ? 1 : 4);  // uint4 per lane per sector
   constexpr int SEC_V = (MODE == 4 || MODE == 5) ? 2 : 4;                    // uint4 per sector
   const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -39,7 +56,7 @@ __global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, u
 #pragma unroll
   for (int r = 0; r < kGatherPerLane; ++r) {
     const uint64_t i = group * kGatherPerLane + r;
-    const uint64_t s = i < n_reads ? sector_of(i, shift) : 0;
+    const uint64_t s = i < n_reads ? (RANDOM ? sector_of_random(i, shift) : sector_of(i, shift)) : 0;
     const u32x4* p = buf + s * SEC_V + part * V;
 #pragma unroll
     for (int j = 0; j < V; ++j) v[r][j] = NT ? __builtin_nontemporal_load(p + j) : p[j];
@@ -54,7 +71,7 @@ __global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, u
 template <int MODE>
 static hipError_t launch_gather(const void* buf, uint32_t shift, uint64_t n, uint64_t* sink,
                                 hipStream_t st) {
-  constexpr int LANES = (MODE == 2 || MODE == 3) ? 4 : (MODE == 5 ? 2 : 1);
+  constexpr int LANES = (MODE == 2 || MODE == 3 || MODE == 6) ? 4 : (MODE == 5 ? 2 : 1);
   const uint64_t threads = (n + kGatherPerLane - 1) / kGatherPerLane * LANES;
   const uint64_t blocks = (threads + 255) / 256;
   hipLaunchKernelGGL(k_gather<MODE>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st,
@@ -68,7 +85,7 @@ extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t 
                                 uint64_t* d_sink, dtl_stream_t stream) {
   using namespace dtlh;
   const uint64_t sector = (mode == 4 || mode == 5) ? 32 : 64;
-  if (!d_buf || !d_sink || mode > 5 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
+  if (!d_buf || !d_sink || mode > 7 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
     return DTL_EINVAL;
   if ((n_reads + kGatherPerLane - 1) / kGatherPerLane * 4 / 256 > 0x7FFFFFFFull) return DTL_EINVAL;
   const uint64_t sectors = buf_bytes / sector;
@@ -83,7 +100,9 @@ extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t 
     case 2: e = launch_gather<2>(d_buf, shift, n_reads, d_sink, st); break;
     case 3: e = launch_gather<3>(d_buf, shift, n_reads, d_sink, st); break;
     case 4: e = launch_gather<4>(d_buf, shift, n_reads, d_sink, st); break;
-    default: e = launch_gather<5>(d_buf, shift, n_reads, d_sink, st); break;
+    case 5: e = launch_gather<5>(d_buf, shift, n_reads, d_sink, st); break;
+    case 6: e = launch_gather<6>(d_buf, shift, n_reads, d_sink, st); break;
+    default: e = launch_gather<7>(d_buf, shift, n_reads, d_sink, st); break;
   }
   if (e != hipSuccess) {
     set_last_error(e);

@@ -2,6 +2,7 @@
 include/dtl_hip.h declares, and its host-side logic (descriptor sizing and
 validation, the cuckoo host build) agrees with the oracle.  No kernel is
 launched here."""
+import ctypes
 import os
 import re
 
@@ -166,3 +167,13 @@ def test_cuckoo_host_build_overflow_sets_victim():
     buf, ins = D.cf_build_host(d, keys, seed=7)
     assert ins < len(keys)
     assert buf[d.victim_offset + 8] == 1
+
+
+def test_gather_bench_rejects_bad_arguments():
+    """Argument checks of the roofline microbenchmark run before any GPU call."""
+    L = D.load()
+    fake = ctypes.c_void_p(4096)
+    sink = ctypes.c_void_p(8192)
+    assert L.dtl_gather_bench(fake, 1 << 20, 8, 1024, sink, None) == D.EINVAL   # modes 0..7
+    assert L.dtl_gather_bench(fake, (1 << 20) + 64, 0, 1024, sink, None) == D.EINVAL  # not a power of two
+    assert L.dtl_gather_bench(None, 1 << 20, 0, 1024, sink, None) == D.EINVAL
