@@ -328,7 +328,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
     return res
 
 
-def run_learned(torch, D, train_fraction=0.25, samples=2000000, epochs=16, model_path=None):
+def run_learned(torch, D, train_fraction=0.25, samples=4000000, epochs=16, model_path=None):
     """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
     bits, timed on the GPU; FPR measured per configuration and bits per key),
     MLP trained on a seeded `train_fraction` of the (configuration, size)

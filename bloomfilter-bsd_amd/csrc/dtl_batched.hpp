@@ -7,7 +7,8 @@
 // an atomic counter until the counter passes the last tile, so large tasks are
 // spread over the whole GPU and small ones do not leave CUs idle.  Filters up
 // to `lds_max` bytes are staged in LDS once per run of tiles of the same task;
-// larger filters are probed from global memory (L2 / Infinity Cache / HBM).
+// larger filters are probed from global memory (L2 / Infinity Cache / HBM),
+// lane-split (split_probe_tile) where the configuration allows.
 // Output is the same bitmap + per-tile count as the single-filter kernels, so
 // the compaction phases are shared.
 #pragma once
@@ -36,6 +37,8 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const W* lds = reinterpret_cast<const W*>(lds_filter);
   uint32_t staged = 0xFFFFFFFFu;
+  // per-lane multipliers of the lane-split path (nothing for other configurations)
+  [[maybe_unused]] const std::conditional_t<C::split.ok, SplitLane<C>, NoSplitLane> sl(lane & (C::split.lanes - 1));
   for (;;) {
     if (tid == 0) s_start = atomicAdd(work_counter, chunk);
     __syncthreads();
@@ -68,6 +71,17 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
       }
       const uint64_t local_base = static_cast<uint64_t>(g - T.tile_begin) * kTileKeys;
       uint32_t cnt = 0;
+      if constexpr (C::split.ok) {
+        if (!use_lds) {
+          // blocks of 32 B or more from L2 / HBM: several lanes per key
+          cnt = split_probe_tile<C, BT>(sl, filter, keys, n, addr, local_base,
+                                        bits + static_cast<uint64_t>(g) * kTileWords);
+          const uint32_t total = block_sum_waves<BT>(cnt, red);
+          if (tid == 0) tile_cnt[g] = total;
+          __syncthreads();
+          continue;
+        }
+      }
 #pragma unroll
       for (int r0 = 0; r0 < ROUNDS; r0 += U) {
         uint32_t key[U];
@@ -79,7 +93,7 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
           valid[u] = i < n;
           key[u] = load_key(keys, valid[u] ? i : n - 1);  // tasks have n >= 1
         }
-        if (use_lds) {
+        if (use_lds || C::split.ok) {
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const uint32_t b = block_of(key[u] * kMul.v[0], addr);

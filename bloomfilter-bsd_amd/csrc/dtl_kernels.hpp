@@ -234,11 +234,8 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
 // The key's hit is the AND of its L lanes' ballot bits (and_compress).
 // Same grid, tile order and outputs as k_bbf_probe_global.
 // ---------------------------------------------------------------------------
-#ifndef DTL_SPLIT_PIPE
-#define DTL_SPLIT_PIPE 0  // 1: double-buffered batches (next batch in flight while one is tested)
-#endif
 #ifndef DTL_SPLIT_UNROLL
-#define DTL_SPLIT_UNROLL (DTL_SPLIT_PIPE ? 8 : 16)  // 16-byte loads per lane per batch
+#define DTL_SPLIT_UNROLL 16  // 16-byte loads per lane per batch (8 and 32 measured slower)
 #endif
 
 // Bit j of the result = AND of bits [j*L, (j+1)*L) of m (64 / L result bits).
@@ -337,36 +334,38 @@ struct SplitLane {
 #define DTL_SPLIT_WAVES 1  // minimum waves per SIMD the register allocation must allow
 #endif
 
-template <class C>
-__global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_split(
-    const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
-    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
-    uint32_t tiles_per_xcd, uint32_t ntiles) {
+// Stand-in for SplitLane in kernels instantiated for every configuration.
+struct NoSplitLane {
+  __device__ __forceinline__ explicit NoSplitLane(uint32_t) {}
+};
+
+// One tile of kTileKeys keys (keys[tile_base ..], n keys in all) through the
+// lane-split test, BT threads: ballot words to bits_tile[0 .. kTileWords),
+// returns the wave's match count (uniform per wave).  `sl` is the lane's
+// SplitLane for piece = lane % L.
+template <class C, int BT>
+__device__ __forceinline__ uint32_t split_probe_tile(const SplitLane<C>& sl, const typename C::word_t* filter,
+                                                     const uint32_t* __restrict__ keys, uint32_t n,
+                                                     const AddrArgs& addr, uint64_t tile_base,
+                                                     uint64_t* __restrict__ bits_tile) {
   using W = typename C::word_t;
   constexpr SplitSched P = C::split;
   static_assert(P.ok, "configuration has no lane-split form");
   constexpr int L = P.lanes;
   constexpr int KPI = 64 / L;                   // keys per wave load instruction
-  constexpr int BT = kGlobalThreads;
   constexpr int ROUNDS = kTileKeys / BT;        // 64-key groups per wave and tile
   constexpr int U = DTL_SPLIT_UNROLL < ROUNDS * L ? DTL_SPLIT_UNROLL : ROUNDS * L;
   constexpr int RU = U / L;                     // 64-key groups per batch
   static_assert(U % L == 0 && ROUNDS % RU == 0, "unroll must cover whole 64-key groups");
-  constexpr uint64_t BYTES = C::w * sizeof(W);
-  __shared__ uint32_t red[BT / 64];
-  const uint32_t tile =
-      tiles_per_xcd ? (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3) : blockIdx.x;
-  if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  constexpr uint32_t BYTES = C::w * sizeof(W);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   // the L lanes of a key are adjacent: the L1 merges a block's pieces only
   // within neighbouring lanes (pieces 64 / L lanes apart measured 1.17x slower)
-  const uint32_t piece = lane & (L - 1), kl = lane / L;
-  const SplitLane<C> sl(piece);
-  // 32-bit byte offsets from the uniform base (the host keeps filters of
-  // this kernel under 4 GiB), so a load is one address op + saddr form.
+  const uint32_t kl = lane / L;
+  // 32-bit byte offsets from the uniform base (filters of this path are
+  // under 4 GiB), so a load is one address op + saddr form.
   const char* fbase = reinterpret_cast<const char*>(filter);
-  const uint32_t poff = piece * 16u;
-  const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
+  const uint32_t poff = (lane & (L - 1)) * 16u;
   uint32_t cnt = 0;
   // The loop is instantiated twice, for POW2 (block = h >> post) and MAGIC
   // addressing; the branch is uniform over the grid.
@@ -378,58 +377,58 @@ __global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_s
       const uint64_t i = tile_base + static_cast<uint64_t>(j) * BT + tid;
       kr[j] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
     }
-    // sub-round s covers keys (s % L) * KPI .. + KPI of round s / L
-    auto issue = [&](int s, uint32_t& key, uint4& dat) {
-      key = __shfl(kr[s / L], (s % L) * KPI + kl);
-      const uint32_t h = key * kMul.v[0];
-      const uint32_t b = decltype(pow2)::value ? h >> addr.post : block_of(h, addr);
-      dat = *reinterpret_cast<const uint4*>(fbase + (b * static_cast<uint32_t>(BYTES) + poff));
-    };
-    auto finish = [&](int round, const uint32_t (&key)[U], const uint4 (&dat)[U], int u0) {
-      const uint64_t g0 = tile_base + static_cast<uint64_t>(round) * BT + wave * 64u;
-      uint64_t word = 0;
-#pragma unroll
-      for (int p = 0; p < L; ++p) word |= and_compress<L>(__ballot(sl.test(key[u0 + p], dat[u0 + p]))) << (p * KPI);
-      // keys past n (last tile only): clear their bits, uniformly per wave
-      if (g0 + 64 > n) word &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
-      if (lane == 0) store_bits(bits, g0 / 64, word);
-      cnt += __popcll(word);
-    };
     constexpr int NB = ROUNDS / RU;  // batches of U sub-rounds
-#if DTL_SPLIT_PIPE
-    // double-buffered: batch b + 1 is in flight while batch b is tested
-    uint32_t key[2][U];
-    uint4 dat[2][U];
-#pragma unroll
-    for (int b = 0; b <= NB; ++b) {
-      if (b < NB) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) issue(b * U + u, key[b & 1][u], dat[b & 1][u]);
-      }
-      if (b > 0) {
-#pragma unroll
-        for (int j = 0; j < RU; ++j) finish((b - 1) * RU + j, key[(b - 1) & 1], dat[(b - 1) & 1], j * L);
-      }
-    }
-#else
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       uint32_t key[U];
       uint4 dat[U];
+      // sub-round s covers keys (s % L) * KPI .. + KPI of round s / L
 #pragma unroll
-      for (int u = 0; u < U; ++u) issue(b * U + u, key[u], dat[u]);
+      for (int u = 0; u < U; ++u) {
+        const int sr = b * U + u;
+        key[u] = __shfl(kr[sr / L], (sr % L) * KPI + kl);
+        const uint32_t h = key[u] * kMul.v[0];
+        const uint32_t blk = decltype(pow2)::value ? h >> addr.post : block_of(h, addr);
+        dat[u] = *reinterpret_cast<const uint4*>(fbase + (blk * BYTES + poff));
+      }
 #pragma unroll
-      for (int j = 0; j < RU; ++j) finish(b * RU + j, key, dat, j * L);
+      for (int j = 0; j < RU; ++j) {
+        const int round = b * RU + j;
+        const uint64_t g0 = tile_base + static_cast<uint64_t>(round) * BT + wave * 64u;
+        uint64_t word = 0;
+#pragma unroll
+        for (int p = 0; p < L; ++p)
+          word |= and_compress<L>(__ballot(sl.test(key[j * L + p], dat[j * L + p]))) << (p * KPI);
+        // keys past n (last tile only): clear their bits, uniformly per wave
+        if (g0 + 64 > n) word &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
+        if (lane == 0) store_bits(bits_tile, static_cast<uint32_t>(round) * (BT / 64) + wave, word);
+        cnt += __popcll(word);
+      }
     }
-#endif
   };
   if (addr.magic == 0) {
     run(std::true_type{});
   } else {
     run(std::false_type{});
   }
+  return cnt;
+}
+
+template <class C>
+__global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_split(
+    const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
+    AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
+    uint32_t tiles_per_xcd, uint32_t ntiles) {
+  constexpr int BT = kGlobalThreads;
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tile =
+      tiles_per_xcd ? (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  const SplitLane<C> sl(threadIdx.x & (C::split.lanes - 1));
+  const uint32_t cnt = split_probe_tile<C, BT>(sl, filter, keys, n, addr, static_cast<uint64_t>(tile) * kTileKeys,
+                                               bits + static_cast<uint64_t>(tile) * kTileWords);
   const uint32_t total = block_sum_waves<BT>(cnt, red);
-  if (tid == 0) tile_cnt[tile] = total;
+  if (threadIdx.x == 0) tile_cnt[tile] = total;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,4 +1,4 @@
-// dtl_mlp.hip -- the 8-64-64-1 FP32 multilayer perceptron of the learned
+// dtl_mlp.hip -- the 10-64-64-1 FP32 multilayer perceptron of the learned
 // configuration model: hand-written training (forward, backward, Adam) and
 // inference kernels.  No reference counterpart: it stands in for the
 // reference's exhaustive calibration + skyline lookup
@@ -21,7 +21,7 @@
 
 namespace dtlh {
 
-constexpr int kIn = 8, kH = 64;
+constexpr int kIn = 10, kH = 64;
 constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
               oB3 = oW3 + kH, kParams = oB3 + 1;
 constexpr int kRows = 64;  // samples per training workgroup

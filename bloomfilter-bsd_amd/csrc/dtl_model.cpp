@@ -105,6 +105,32 @@ double table_cost(const dtl_cost_model& M, int c, double m, double n, double tw)
   return interp_t(M, c, std::log2(m)) + tw * interp_fpr(M, c, m / n);
 }
 
+// Inputs of the MLP: the configuration (word size, w, s, z, k, block bytes
+// and the lanes per key of its lane-split probe kernel, which changes its
+// lookup cost in steps), filter size, bits per key and tw.
+constexpr int kFeat = 10;
+
+uint32_t ilog2(uint32_t v) { return 31u - static_cast<uint32_t>(__builtin_clz(v)); }
+
+// Lanes per key of the lane-split kernel, looked up once per configuration
+// (thread-safe static initialisation; features() runs on several threads).
+uint32_t split_lanes_of(const Cfg& c) {
+  auto key = [](uint32_t wb, uint32_t w, uint32_t s, uint32_t z, uint32_t k) {
+    return ((wb == 64 ? 1u : 0u) << 13) | (ilog2(w) << 10) | (ilog2(s) << 7) | (ilog2(z) << 5) | (k & 31u);
+  };
+  static const std::vector<uint8_t> table = [&] {
+    std::vector<uint8_t> t(1u << 14, 0);
+    for (uint32_t wb : {32u, 64u})
+      for (uint32_t w = 1; w <= 16; w *= 2)
+        for (uint32_t s = 1; s <= 16; s *= 2)
+          for (uint32_t z = 1; z <= 8; z *= 2)
+            for (uint32_t k = 1; k <= 16; ++k)
+              t[key(wb, w, s, z, k)] = static_cast<uint8_t>(dtl_bbf_split_lanes(wb, w, s, z, k));
+    return t;
+  }();
+  return table[key(c.word_bits, c.w, c.s, c.z, c.k)];
+}
+
 void features(const Cfg& c, double m, double n, double tw, float* f) {
   f[0] = c.word_bits / 64.0f;
   f[1] = static_cast<float>(std::log2(c.w) / 4.0);
@@ -114,6 +140,8 @@ void features(const Cfg& c, double m, double n, double tw, float* f) {
   f[5] = static_cast<float>((std::log2(m) - 23.0) / 10.0);
   f[6] = static_cast<float>(std::log2(m / n) - 3.3);  // bits per key
   f[7] = static_cast<float>(std::log2(tw) / 10.0);
+  f[8] = static_cast<float>(std::log2(c.word_bits / 8.0 * c.w) / 7.0);  // block bytes
+  f[9] = split_lanes_of(c) / 8.0f;
 }
 
 struct DevBuf {
@@ -304,7 +332,7 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
     return s;
   };
   auto unif = [&]() { return (next() >> 11) * (1.0 / 9007199254740992.0); };
-  r.X.assign(static_cast<size_t>(count) * 8, 0.f);
+  r.X.assign(static_cast<size_t>(count) * kFeat, 0.f);
   r.Y.assign(count, 0.f);
   r.W.assign(count, 0.f);
   std::vector<int> s_si(count);
@@ -315,7 +343,7 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
     const double m = std::ldexp(1.0, M.log2_min + si);
     const double bpk = 4.0 * std::pow(5.0, unif());
     const double tw = std::pow(1000.0, unif());
-    features(M.cfgs[c], m, m / bpk, tw, &r.X[static_cast<size_t>(i) * 8]);
+    features(M.cfgs[c], m, m / bpk, tw, &r.X[static_cast<size_t>(i) * kFeat]);
     r.Y[i] = static_cast<float>(std::log(M.t_ns[cell] + tw * interp_fpr(M, c, bpk)));
     s_si[i] = si;
     s_bpk[i] = bpk;
@@ -389,7 +417,7 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
           if (!M->train_cell[static_cast<size_t>(c) * ns + si]) continue;
           const double m = std::ldexp(1.0, M->log2_min + si), bpk = m / n;
           if (bpk < 4.0 || bpk > 20.0) continue;
-          float f[8];
+          float f[kFeat];
           features(M->cfgs[c], m, n, tw, f);
           qx.insert(qx.end(), f, f + 8);
           qcost.push_back(M->t_ns[static_cast<size_t>(c) * ns + si] + tw * interp_fpr(*M, c, bpk));
@@ -476,8 +504,8 @@ int dtl_model_choose(const dtl_cost_model* M, uint64_t n, double tw_ns, int use_
   std::vector<double> score(nk);
   if (use_mlp) {
     hipStream_t st = static_cast<hipStream_t>(stream);
-    std::vector<float> X(nk * 8), Y(nk);
-    for (size_t i = 0; i < nk; ++i) features(M->cfgs[cand_c[i]], cand_m[i], static_cast<double>(n), tw_ns, &X[i * 8]);
+    std::vector<float> X(nk * kFeat), Y(nk);
+    for (size_t i = 0; i < nk; ++i) features(M->cfgs[cand_c[i]], cand_m[i], static_cast<double>(n), tw_ns, &X[i * kFeat]);
     DevBuf dx, dy;
     if (!dx.alloc(X.size() * 4) || !dy.alloc(nk * 4)) return DTL_ENOMEM;
     if (hipMemcpyAsync(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) return DTL_EHIP;
@@ -516,11 +544,13 @@ double dtl_model_cost(const dtl_cost_model* M, const dtl_bbf_choice* ch, uint64_
 }
 
 // Persistence: a small binary file (magic, version, grid, tables, MLP weights).
+// Version 2: 10 MLP inputs.
+constexpr uint32_t kModelVersion = 2;
 int dtl_model_save(const dtl_cost_model* M, const char* path) {
   if (!M || !path) return DTL_EINVAL;
   FILE* f = std::fopen(path, "wb");
   if (!f) return DTL_ERUNTIME;
-  const uint32_t hdr[5] = {0x4C544448u /* "HDTL" */, 1u, static_cast<uint32_t>(M->cfgs.size()), M->log2_min,
+  const uint32_t hdr[5] = {0x4C544448u /* "HDTL" */, kModelVersion, static_cast<uint32_t>(M->cfgs.size()), M->log2_min,
                            M->log2_max};
   bool ok = std::fwrite(hdr, sizeof(hdr), 1, f) == 1;
   ok = ok && std::fwrite(M->cfgs.data(), sizeof(Cfg), M->cfgs.size(), f) == M->cfgs.size();
@@ -546,7 +576,7 @@ int dtl_model_load(const char* path, dtl_cost_model** out) {
   if (!f) return DTL_ERUNTIME;
   uint32_t hdr[5];
   dtl_cost_model* M = new dtl_cost_model();
-  bool ok = std::fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == 0x4C544448u && hdr[1] == 1u && hdr[2] < 100000 &&
+  bool ok = std::fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == 0x4C544448u && hdr[1] == kModelVersion && hdr[2] < 100000 &&
             hdr[3] <= hdr[4] && hdr[4] <= 40;
   if (ok) {
     M->cfgs.resize(hdr[2]);

@@ -325,7 +325,7 @@ def gather_bench(buf, mode, n_reads, sink, stream=None):
 
 
 class MLP:
-    """The library's FP32 8-64-64-1 MLP (dtl_mlp.hip)."""
+    """The library's FP32 10-64-64-1 MLP (dtl_mlp.hip)."""
 
     def __init__(self, seed=0):
         self.L = load()
@@ -399,7 +399,7 @@ class CostModel:
     def save(self, path):
         _check("dtl_model_save", self.L.dtl_model_save(self.h, path.encode()))
 
-    def train(self, train_fraction=0.25, samples=200000, epochs=40, seed=1):
+    def train(self, train_fraction=0.25, samples=4000000, epochs=16, seed=1):
         _check("dtl_model_train", self.L.dtl_model_train(self.h, train_fraction, samples, epochs, seed,
                                                          _stream(None)))
 

@@ -504,7 +504,7 @@ class cost_model {
   /// Runs the calibration sweep on the current GPU (params = nullptr: the
   /// full sweep, 2^13..2^33 bits) and, if `train`, trains the MLP on 25% of
   /// the calibrated cells.
-  static cost_model calibrate(const dtl_calib_params* params = nullptr, bool train = true, u32 samples = 2000000,
+  static cost_model calibrate(const dtl_calib_params* params = nullptr, bool train = true, u32 samples = 4000000,
                               u32 epochs = 16) {
     dtl_cost_model* p = nullptr;
     detail::throw_status(dtl_model_calibrate(params, &p, nullptr), "calibration");

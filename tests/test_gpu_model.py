@@ -1,7 +1,7 @@
 """The learned configuration model on the GPU.
 
 * The FP32 MLP's hand-written inference and gradient kernels against a plain
-  PyTorch float64 restatement of the same 8-64-64-1 network (tolerances are
+  PyTorch float64 restatement of the same 10-64-64-1 network (tolerances are
   written in each test; the kernels accumulate in FP32, the gradient with
   float atomics in an unspecified order).
 * Training reduces the loss.
@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 import dtl_hip as D  # noqa: E402
 
-KIN, KH = 8, 64
+KIN, KH = 10, 64
 
 
 def dev():

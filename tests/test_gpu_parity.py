@@ -517,3 +517,12 @@ def test_batched_mixed_configurations():
         for g, e in zip(plan.results(), expect):
             assert np.array_equal(g, e)
     plan.close()
+    # every filter from global memory: lane-split groups (u64 w8 s8, zoned u64
+    # w8 z4) and one-lane groups, 256- and 1024-thread workgroups
+    for threads in (256, 1024):
+        plan = D.BatchPlan(tasks, D.ProbeParams(0, threads, 0, 2, D.PROBE_FORCE_GLOBAL))
+        plan.run()
+        torch.cuda.synchronize()
+        for g, e in zip(plan.results(), expect):
+            assert np.array_equal(g, e)
+        plan.close()
