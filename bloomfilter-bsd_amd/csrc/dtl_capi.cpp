@@ -425,7 +425,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
                      &scratch);
     if (rc != DTL_OK) return rc;
     char* sp = static_cast<char*>(scratch);
-    BbfPartLaunch P;
+    BbfPartLaunch P{};
     P.words = d_words;
     P.keys = d_keys;
     P.n = static_cast<uint32_t>(n);
@@ -513,6 +513,12 @@ int bbf_probe(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys
 
 int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
              uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream) {
+  return cf_probe_ex(d, d_buf, d_keys, n, match_offset, d_pos, d_count, nullptr, stream);
+}
+
+int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
+                uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, const dtl_probe_params* params,
+                dtl_stream_t stream) {
   if (!d || !d_buf || !d_count || n > 0xFFFFFFFFull || d->bucket_cnt < 2) return DTL_EINVAL;
   if (n && (!d_keys || !d_pos)) return DTL_EINVAL;
   if (!aligned_to(d_buf, 16)) return DTL_EINVAL;
@@ -528,6 +534,116 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   const uint32_t ntiles = static_cast<uint32_t>((n + kTileKeys - 1) / kTileKeys);
   const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
   const size_t cnt_bytes = (static_cast<size_t>(ntiles) + kScanTmpWords) * 4;
+  const uint32_t flags = params ? params->flags : 0u;
+  auto ev = [&](void* evp) {
+    return !evp || hipEventRecord(static_cast<hipEvent_t>(evp), st) == hipSuccess;
+  };
+
+  // partitioned probe: two passes over keys grouped by bucket slice
+  const uint64_t table_bytes = d->victim_offset;
+  const bool want = !(flags & DTL_PROBE_DIRECT) &&
+                    ((flags & DTL_PROBE_PARTITIONED) || (table_bytes >= (64ull << 20) && n >= (1ull << 22)));
+  uint32_t slice_shift = 0, nparts = 0;
+  if (want) {
+    const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
+                                : table_bytes >= (512ull << 20) ? 21u
+                                : table_bytes >= (128ull << 20) ? 20u : 19u;
+    const uint32_t bb_log2 = floor_log2(d->bytes_per_bucket);
+    slice_shift = slice_log2 > bb_log2 ? slice_log2 - bb_log2 : 0u;
+    // buckets 0 .. bucket_cnt: a MAGIC alternate index can equal bucket_cnt (it reads the victim bytes)
+    const uint64_t nb = static_cast<uint64_t>(d->bucket_cnt) + 1;
+    while (slice_shift < 31 && ((nb + (1ull << slice_shift) - 1) >> slice_shift) > kPartMax) ++slice_shift;
+    nparts = static_cast<uint32_t>((nb + (1ull << slice_shift) - 1) >> slice_shift);
+    if (nparts < 2) nparts = 0;
+  }
+  if (nparts) {
+    const uint32_t bucket_shift = hit_bucket_shift(ntiles);
+    const uint32_t bucket_tiles = (1u << bucket_shift) / kTileKeys;
+    const uint32_t nbuckets = (ntiles + bucket_tiles - 1) / bucket_tiles;
+    const uint32_t hit_groups = std::min<uint32_t>(kHitGroupsMax, 2u * static_cast<uint32_t>(std::max(di.cu_count, 1)));
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t cnt_r = up(cnt_bytes);
+    const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
+    const size_t part_bytes = up((nbins * 3 + 1) * 4);
+    const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
+    const size_t matrix_bytes = up(static_cast<size_t>(hit_groups) * nbuckets * 4);
+    const size_t key_bytes = up(static_cast<size_t>(n) * 4);
+    // a key can match in both passes: match positions take up to 2n words
+    const size_t hit_bytes = up(static_cast<size_t>(n) * 8);
+    void* scratch = nullptr;
+    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + 256 + matrix_bytes +
+                                  key_bytes + 2 * hit_bytes, &scratch);
+    if (rc != DTL_OK) return rc;
+    char* sp = static_cast<char*>(scratch);
+    BbfPartLaunch P{};
+    P.words = d_buf;
+    P.keys = d_keys;
+    P.n = static_cast<uint32_t>(n);
+    if (d->addr_mode == DTL_ADDR_POW2) {
+      P.addr_post = d->bucket_shift;
+    } else {
+      P.addr_magic = d->magic;
+      P.addr_mshift = d->magic_shift;
+      P.addr_divisor = d->bucket_cnt;
+    }
+    P.slice_shift = slice_shift;
+    P.nparts = nparts;
+    P.tag_mask = static_cast<uint32_t>((1ull << d->tag_bits) - 1ull);
+    P.cf_mode = d->addr_mode;
+    P.cf_bucket_mask = d->bucket_mask;
+    P.cf_bucket_cnt = d->bucket_cnt;
+    P.cf_victim_offset = d->victim_offset;
+    P.scatter_threads = (params && params->block_threads) ? params->block_threads : 1024u;
+    P.bits = reinterpret_cast<uint64_t*>(sp);
+    sp += bits_bytes;
+    P.tile_cnt = reinterpret_cast<uint32_t*>(sp);
+    sp += cnt_r;
+    P.pbits = reinterpret_cast<uint64_t*>(sp);
+    sp += bits_bytes;
+    P.ptile_cnt = reinterpret_cast<uint32_t*>(sp);
+    sp += cnt_r;
+    P.counts = reinterpret_cast<uint32_t*>(sp);
+    P.offsets = P.counts + nbins;
+    P.cursor = P.offsets + nbins + 1;
+    sp += part_bytes;
+    P.d_hits = reinterpret_cast<uint64_t*>(sp);
+    P.bucket_shift = bucket_shift;
+    P.nbuckets = nbuckets;
+    P.hit_groups = hit_groups;
+    P.bcounts = reinterpret_cast<uint32_t*>(sp + 8);
+    P.boffsets = P.bcounts + nbuckets;
+    P.bcursor = P.boffsets + nbuckets + 1;
+    sp += bucket_bytes;
+    uint64_t* d_h1 = reinterpret_cast<uint64_t*>(sp);
+    uint64_t* d_h2 = d_h1 + 1;
+    sp += 256;
+    P.hcounts = reinterpret_cast<uint32_t*>(sp);
+    sp += matrix_bytes;
+    P.keys_p = reinterpret_cast<uint32_t*>(sp);
+    sp += key_bytes;
+    P.pos_p = reinterpret_cast<uint32_t*>(sp);  // grouped positions, later the bucketed matches (<= 2n)
+    sp += hit_bytes;
+    uint32_t* hp = reinterpret_cast<uint32_t*>(sp);  // matches of both passes (<= 2n)
+    P.ntiles = ntiles;
+    P.split = 0;
+    P.cu_count = di.cu_count;
+    P.stream = st;
+    if (!ev(params ? params->ev_start : nullptr)) return DTL_EHIP;
+    hipError_t he = hipSuccess;
+    for (uint32_t pass = 1; pass <= 2 && he == hipSuccess; ++pass) {
+      P.part_kind = pass - 1;
+      P.cf_which = pass;
+      he = partition_keys(P);
+      if (he == hipSuccess) he = e->probe_part(P);
+      if (he == hipSuccess) he = part_gather_matches(P, hp, pass == 1 ? nullptr : d_h1, pass == 1 ? d_h1 : d_h2);
+    }
+    if (he == hipSuccess) he = add_counts(d_h1, d_h2, P.d_hits, st);
+    if (he == hipSuccess) he = part_sort_matches(P, hp, P.d_hits);
+    if (he != hipSuccess) return hip_status(he);
+    if (!ev(params ? params->ev_stop : nullptr)) return DTL_EHIP;
+    return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));
+  }
+
   void* scratch = nullptr;
   rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
   if (rc != DTL_OK) return rc;
@@ -546,8 +662,10 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   L.tile_cnt = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + bits_bytes);
   L.ntiles = ntiles;
   L.stream = st;
+  if (!ev(params ? params->ev_start : nullptr)) return DTL_EHIP;
   hipError_t he = e->probe(L);
   if (he != hipSuccess) return hip_status(he);
+  if (!ev(params ? params->ev_stop : nullptr)) return DTL_EHIP;
   return hip_status(compact_single(L.bits, L.tile_cnt, ntiles, L.n, match_offset, d_pos, d_count, st));
 }
 

@@ -181,16 +181,19 @@ static uint32_t expand_tiles(uint32_t ntiles) {
   return ntiles >= 65536 ? 8u : ntiles >= 16384 ? 4u : ntiles >= 4096 ? 2u : 1u;
 }
 
-// Writes src[i] (a gathered value) for every match i instead of i itself.
+// Writes src[i] (a gathered value) for every match i instead of i itself,
+// at out[*d_base + rank] (d_base null: at out[rank]).
 __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ offs,
                                                                   uint32_t ntiles, uint32_t per_wg,
                                                                   const uint32_t* __restrict__ src,
-                                                                  uint32_t* __restrict__ out) {
+                                                                  uint32_t* __restrict__ out,
+                                                                  const uint64_t* __restrict__ d_base) {
   __shared__ uint64_t sh[kExpandThreads / 64];
   const uint32_t t0 = blockIdx.x * per_wg;
+  uint32_t* dst = out + (d_base ? *d_base : 0ull);
   expand_group(bits, t0, min(per_wg, ntiles - t0), offs, sh, [&](uint32_t tile, uint32_t i, uint32_t o) {
-    out[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i];
+    dst[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i];
   });
 }
 
@@ -240,10 +243,17 @@ hipError_t scan_tile_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_tot
 }
 
 hipError_t expand_gather(const uint64_t* bits, const uint32_t* offs, uint32_t ntiles, const uint32_t* src,
-                         uint32_t* out, hipStream_t stream) {
+                         uint32_t* out, const uint64_t* d_out_base, hipStream_t stream) {
   const uint32_t per = expand_tiles(ntiles);
   hipLaunchKernelGGL(k_expand_gather, dim3((ntiles + per - 1) / per), dim3(kExpandThreads), 0, stream, bits, offs,
-                     ntiles, per, src, out);
+                     ntiles, per, src, out, d_out_base);
+  return hipGetLastError();
+}
+
+__global__ void k_add_counts(const uint64_t* a, const uint64_t* b, uint64_t* out) { *out = *a + *b; }
+
+hipError_t add_counts(const uint64_t* a, const uint64_t* b, uint64_t* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_add_counts, dim3(1), dim3(1), 0, stream, a, b, out);
   return hipGetLastError();
 }
 

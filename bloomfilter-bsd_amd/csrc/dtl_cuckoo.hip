@@ -9,74 +9,11 @@
 // Output is the shared bitmap + tile-count format (dtl_kernels.hpp).
 #include <hip/hip_runtime.h>
 
+#include "dtl_cuckoo.hpp"
 #include "dtl_internal.hpp"
 #include "dtl_kernels.hpp"
 
 namespace dtlh {
-
-struct CfArgs {
-  uint32_t magic, mshift, divisor, post;  // bucket addressing, as AddrArgs
-  uint32_t mode;                          // 0 = POW2, 1 = MAGIC
-  uint32_t bucket_mask;                   // POW2
-  uint32_t bucket_cnt;                    // MAGIC
-  uint32_t pad;
-  uint64_t victim_offset;
-};
-
-// alt_index (cuckoofilter_logic.hpp:214-252)
-__device__ __forceinline__ uint32_t cf_alt(uint32_t i, uint32_t tag, const CfArgs& a) {
-  const uint32_t s = tag * 0x5bd1e995u;
-  if (a.mode == 0) return (i ^ s) & a.bucket_mask;
-  const uint32_t q = __umulhi(s, a.magic) >> a.mshift;
-  const uint32_t s_mod_m = s - q * a.divisor;
-  const uint32_t x = a.bucket_cnt - s_mod_m;
-  const uint32_t mask = (x < i) ? 0xFFFFFFFFu : 0u;
-  return (mask & a.bucket_cnt) + (x - i);
-}
-
-// Little-endian bits of bucket i (bytes_per_bucket = (T*B+7)/8 <= 16).
-template <int T, int B>
-struct Bucket {
-  static constexpr int BPB = (T * B + 7) / 8;
-  uint64_t lo, hi;
-  __device__ __forceinline__ void load(const uint8_t* __restrict__ base, uint32_t i) {
-    const uint64_t off = static_cast<uint64_t>(i) * BPB;
-    hi = 0;
-    if constexpr (BPB == 16) {
-      const uint4 v = *reinterpret_cast<const uint4*>(base + off);
-      lo = (static_cast<uint64_t>(v.y) << 32) | v.x;
-      hi = (static_cast<uint64_t>(v.w) << 32) | v.z;
-    } else if constexpr (BPB == 8) {
-      const uint2 v = *reinterpret_cast<const uint2*>(base + off);
-      lo = (static_cast<uint64_t>(v.y) << 32) | v.x;
-    } else if constexpr (BPB == 4) {
-      lo = *reinterpret_cast<const uint32_t*>(base + off);
-    } else if constexpr (BPB == 2) {
-      lo = *reinterpret_cast<const uint16_t*>(base + off);
-    } else if constexpr (BPB == 1) {
-      lo = base[off];
-    } else {
-      // 3- or 6-byte buckets: two aligned dwords cover the bucket.
-      const uint64_t a = off & ~static_cast<uint64_t>(3);
-      const uint32_t sh = static_cast<uint32_t>(off & 3u) * 8u;
-      const uint32_t* p = reinterpret_cast<const uint32_t*>(base + a);
-      const uint64_t v = (static_cast<uint64_t>(p[1]) << 32) | p[0];
-      lo = (v >> sh) & ((1ull << (BPB * 8)) - 1ull);
-    }
-  }
-  __device__ __forceinline__ bool has(uint32_t tag) const {
-    bool f = false;
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int bit = T * j;
-      const uint64_t word = bit < 64 ? lo : hi;
-      const int sh = bit & 63;
-      const uint32_t slot = static_cast<uint32_t>(word >> sh) & static_cast<uint32_t>((1ull << T) - 1ull);
-      f |= slot == tag;
-    }
-    return f;
-  }
-};
 
 template <int T, int B>
 __global__ __launch_bounds__(kGlobalThreads) void k_cf_probe(const uint8_t* __restrict__ buf,
@@ -113,8 +50,7 @@ __global__ __launch_bounds__(kGlobalThreads) void k_cf_probe(const uint8_t* __re
       valid[u] = i < n;
       const uint32_t key = load_key(keys, valid[u] ? i : n - 1);
       i1[u] = block_of(key * kMul.v[0], addr);
-      uint32_t t = (key * kMul.v[1]) & kMask;
-      tag[u] = t + (t == 0);
+      tag[u] = cf_tag(key, kMask);
       i2[u] = cf_alt(i1[u], tag[u], a);
     }
 #pragma unroll
@@ -161,13 +97,90 @@ hipError_t launch_cf_probe(const CfProbeLaunch& L) {
   return hipGetLastError();
 }
 
+// One pass of the partitioned cuckoo probe over keys grouped by the slice of
+// one of their buckets (XCD-ordered tiles, as k_bbf_probe_split): pass 1
+// (which = 1, keys grouped by first bucket) tests the first bucket and the
+// victim cache, pass 2 (keys grouped by alternate bucket) the alternate
+// bucket.  A key is found iff one of its two passes finds it, as in
+// cuckoofilter_logic.hpp:346-362.
+template <int T, int B>
+__global__ __launch_bounds__(kGlobalThreads) void k_cf_probe_part(const uint8_t* __restrict__ buf,
+                                                                  const uint32_t* __restrict__ keys,
+                                                                  uint32_t n, CfArgs a, uint32_t which,
+                                                                  uint64_t* __restrict__ bits,
+                                                                  uint32_t* __restrict__ tile_cnt,
+                                                                  uint32_t tiles_per_xcd, uint32_t ntiles) {
+  constexpr int BT = kGlobalThreads;
+  constexpr int ROUNDS = kTileKeys / BT;
+  constexpr int U = ROUNDS;  // every key of the tile in flight (one bucket load each)
+  constexpr uint32_t kMask = static_cast<uint32_t>((1ull << T) - 1ull);
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tile = (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3);
+  if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t* vp = reinterpret_cast<const uint32_t*>(buf + a.victim_offset);
+  const uint32_t v_index = vp[0];
+  const uint32_t v_tag = vp[1];
+  const bool v_used = which == 1 && (vp[2] & 0xFFu) != 0;
+  AddrArgs addr;
+  addr.magic = a.magic;
+  addr.mshift = a.mshift;
+  addr.divisor = a.divisor;
+  addr.post = a.post;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
+  uint32_t cnt = 0;
+  uint32_t i1[U], i2[U], tag[U];
+  Bucket<T, B> bk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t i = tile_base + static_cast<uint64_t>(u) * BT + tid;
+    const uint32_t key = load_key(keys, i < n ? i : n - 1);
+    i1[u] = block_of(key * kMul.v[0], addr);
+    tag[u] = cf_tag(key, kMask);
+    i2[u] = cf_alt(i1[u], tag[u], a);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) bk[u].load(buf, which == 1 ? i1[u] : i2[u]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t g0 = tile_base + static_cast<uint64_t>(u) * BT + wave * 64u;
+    bool hit = v_used & (tag[u] == v_tag) & ((i1[u] == v_index) | (i2[u] == v_index));
+    hit |= bk[u].has(tag[u]);
+    uint64_t m = __ballot(hit);
+    if (g0 + 64 > n) m &= g0 >= n ? 0ull : (~0ull >> (64 - (n - g0)));
+    if (lane == 0) store_bits(bits, g0 / 64, m);
+    cnt += __popcll(m);
+  }
+  const uint32_t total = block_sum_waves<BT>(cnt, red);
+  if (tid == 0) tile_cnt[tile] = total;
+}
+
+template <int T, int B>
+hipError_t launch_cf_probe_part(const BbfPartLaunch& L) {
+  CfArgs a;
+  a.magic = L.addr_magic;
+  a.mshift = L.addr_mshift;
+  a.divisor = L.addr_divisor;
+  a.post = L.addr_post;
+  a.mode = L.cf_mode;
+  a.bucket_mask = L.cf_bucket_mask;
+  a.bucket_cnt = L.cf_bucket_cnt;
+  a.pad = 0;
+  a.victim_offset = L.cf_victim_offset;
+  const uint32_t per_xcd = (L.ntiles + 7u) / 8u;
+  hipLaunchKernelGGL((k_cf_probe_part<T, B>), dim3(per_xcd * 8u), dim3(kGlobalThreads), 0, L.stream,
+                     static_cast<const uint8_t*>(L.words), L.keys_p, L.n, a, L.cf_which, L.pbits, L.ptile_cnt,
+                     per_xcd, L.ntiles);
+  return hipGetLastError();
+}
+
+#define DTL_CF_ENTRY(t, b) {t, b, &launch_cf_probe<t, b>, &launch_cf_probe_part<t, b>}
 static const CfEntry kCfEntries[] = {
-    {4, 1, &launch_cf_probe<4, 1>},   {4, 2, &launch_cf_probe<4, 2>},   {4, 4, &launch_cf_probe<4, 4>},
-    {8, 1, &launch_cf_probe<8, 1>},   {8, 2, &launch_cf_probe<8, 2>},   {8, 4, &launch_cf_probe<8, 4>},
-    {12, 1, &launch_cf_probe<12, 1>}, {12, 2, &launch_cf_probe<12, 2>}, {12, 4, &launch_cf_probe<12, 4>},
-    {16, 1, &launch_cf_probe<16, 1>}, {16, 2, &launch_cf_probe<16, 2>}, {16, 4, &launch_cf_probe<16, 4>},
-    {32, 1, &launch_cf_probe<32, 1>}, {32, 2, &launch_cf_probe<32, 2>}, {32, 4, &launch_cf_probe<32, 4>},
+    DTL_CF_ENTRY(4, 1),  DTL_CF_ENTRY(4, 2),  DTL_CF_ENTRY(4, 4),  DTL_CF_ENTRY(8, 1),  DTL_CF_ENTRY(8, 2),
+    DTL_CF_ENTRY(8, 4),  DTL_CF_ENTRY(12, 1), DTL_CF_ENTRY(12, 2), DTL_CF_ENTRY(12, 4), DTL_CF_ENTRY(16, 1),
+    DTL_CF_ENTRY(16, 2), DTL_CF_ENTRY(16, 4), DTL_CF_ENTRY(32, 1), DTL_CF_ENTRY(32, 2), DTL_CF_ENTRY(32, 4),
 };
+#undef DTL_CF_ENTRY
 
 const CfEntry* cf_find(uint32_t tag_bits, uint32_t tags_per_bucket) {
   for (const CfEntry& e : kCfEntries)

@@ -70,13 +70,20 @@ struct BbfBatchLaunch {
   hipStream_t stream;
 };
 
-// Partitioned probe of one large filter (dtl_partition.hip).
+// Partitioned probe of one large filter (dtl_partition.hip), blocked Bloom
+// or cuckoo.
 struct BbfPartLaunch {
   const void* words;
   const uint32_t* keys;
   uint32_t n;
   uint32_t addr_magic, addr_mshift, addr_divisor, addr_post;
-  uint32_t slice_shift;   // partition = block index >> slice_shift
+  uint32_t slice_shift;   // partition = block (or bucket) index >> slice_shift
+  // partition function: 0 = block / first cuckoo bucket of the key;
+  // 1 = alternate cuckoo bucket (cf_* fields, tag_mask)
+  uint32_t part_kind;
+  uint32_t tag_mask, cf_mode, cf_bucket_mask, cf_bucket_cnt;
+  uint32_t cf_which;         // cuckoo probe pass: 1 = first bucket + victim, 2 = alternate bucket
+  uint64_t cf_victim_offset;
   uint32_t nparts;
   uint32_t scatter_threads;  // 256, 512 or 1024 (tile = 16 keys per thread)
   uint32_t* counts;       // nparts * kPartLanes bins
@@ -130,6 +137,12 @@ constexpr uint32_t kPartLanes = 8;    // sub-ranges per partition (one per XCD)
 // (partitioned-order bitmap -> positions -> key-order bitmap + tile counts).
 hipError_t partition_keys(const BbfPartLaunch& L);
 hipError_t part_reorder_matches(const BbfPartLaunch& L);
+// The two halves of part_reorder_matches: positions of the grouped-order
+// matches appended to hp at *d_base (0 if null), their number to *d_count;
+// then the hit sort of hp[0 .. *d_hits) into the key-order bitmap.
+hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint64_t* d_base, uint64_t* d_count);
+hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits);
+hipError_t add_counts(const uint64_t* a, const uint64_t* b, uint64_t* out, hipStream_t stream);
 
 struct CfProbeLaunch {
   const void* buf;
@@ -146,6 +159,7 @@ struct CfProbeLaunch {
 struct CfEntry {
   uint32_t tag_bits, tags_per_bucket;
   hipError_t (*probe)(const CfProbeLaunch&);
+  hipError_t (*probe_part)(const BbfPartLaunch&);  // one pass over grouped keys
 };
 
 // Registries (filled once, on first use).
@@ -188,9 +202,9 @@ hipError_t write_zero_count(uint64_t* d_count, hipStream_t stream);
 // Exclusive scan of tile counts in place, total to *d_total.
 hipError_t scan_tile_counts(uint32_t* tile_cnt, uint32_t ntiles, uint64_t* d_total, hipStream_t stream);
 // For every set bit i of the bitmap (tile offsets from scan_tile_counts):
-// out[rank] = src[i], in bit order.
+// out[base + rank] = src[i], in bit order (base = *d_out_base, or 0 if null).
 hipError_t expand_gather(const uint64_t* bits, const uint32_t* offs, uint32_t ntiles, const uint32_t* src,
-                         uint32_t* out, hipStream_t stream);
+                         uint32_t* out, const uint64_t* d_out_base, hipStream_t stream);
 hipError_t zero_u32(uint32_t* p, hipStream_t stream);
 
 // Raises a kernel's dynamic-LDS limit when `bytes` exceeds the default

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 
+#include "dtl_cuckoo.hpp"
 #include "dtl_internal.hpp"
 #include "dtl_kernels.hpp"
 #include "dtl_scan.hpp"
@@ -50,8 +51,40 @@ static_assert(kScatterKpt == 16 || kScatterKpt == 32, "16 or 32 keys per thread"
 
 constexpr int kPartScanThreads = 1024;
 
-__device__ __forceinline__ uint32_t part_of(uint32_t key, const AddrArgs& a, uint32_t slice_shift) {
-  return block_of(key * kMul.v[0], a) >> slice_shift;
+// Partition function: partition = bucket >> slice_shift, where bucket is the
+// key's block (blocked Bloom) or first bucket (cuckoo: the same block_of of
+// h0), or (kind 1) the key's alternate cuckoo bucket.  The kind is uniform
+// over a launch.
+struct PartFn {
+  AddrArgs a;
+  uint32_t slice_shift;
+  uint32_t kind;
+  uint32_t tag_mask;
+  CfArgs cf;
+};
+
+__device__ __forceinline__ uint32_t part_of(uint32_t key, const PartFn& f) {
+  uint32_t b = block_of(key * kMul.v[0], f.a);
+  if (f.kind) b = cf_alt(b, cf_tag(key, f.tag_mask), f.cf);
+  return b >> f.slice_shift;
+}
+
+static PartFn make_part_fn(const BbfPartLaunch& L) {
+  PartFn f;
+  f.a = AddrArgs{L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post};
+  f.slice_shift = L.slice_shift;
+  f.kind = L.part_kind;
+  f.tag_mask = L.tag_mask;
+  f.cf.magic = L.addr_magic;
+  f.cf.mshift = L.addr_mshift;
+  f.cf.divisor = L.addr_divisor;
+  f.cf.post = L.addr_post;
+  f.cf.mode = L.cf_mode;
+  f.cf.bucket_mask = L.cf_bucket_mask;
+  f.cf.bucket_cnt = L.cf_bucket_cnt;
+  f.cf.pad = 0;
+  f.cf.victim_offset = L.cf_victim_offset;
+  return f;
 }
 
 __global__ void k_fill_zero_u32(uint32_t* __restrict__ p, uint64_t count) {
@@ -77,7 +110,7 @@ __device__ __forceinline__ uint32_t lane_of(uint64_t i, uint32_t tile_log2) {
 // the bins are lane-major (lane * nparts + partition): the keys of one wave
 // share a lane, so their atomics spread over all banks.
 __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                            AddrArgs a, uint32_t slice_shift,
+                                                            PartFn pf,
                                                             uint32_t nparts, uint32_t per_wg,
                                                             uint32_t tile_log2,
                                                             uint32_t* __restrict__ counts) {
@@ -100,7 +133,7 @@ __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __re
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
       const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
-      if (valid[u]) atomicAdd(&hist[lane_of(i, tile_log2) * nparts + part_of(key[u], a, slice_shift)], 1u);
+      if (valid[u]) atomicAdd(&hist[lane_of(i, tile_log2) * nparts + part_of(key[u], pf)], 1u);
     }
   }
   __syncthreads();
@@ -143,8 +176,7 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
 // threads).
 template <int kScatterThreads>
 __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt)) void k_part_scatter(const uint32_t* __restrict__ keys,
-                                                                  uint32_t n, AddrArgs a,
-                                                                  uint32_t slice_shift, uint32_t nparts,
+                                                                  uint32_t n, PartFn pf, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
                                                                   uint32_t* __restrict__ keys_p,
                                                                   uint32_t* __restrict__ pos_p) {
@@ -169,7 +201,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
-    const uint32_t pid = part_of(key[r], a, slice_shift);
+    const uint32_t pid = part_of(key[r], pf);
     // rank in the partition (low 16 bits, tile < 2^16) and partition id
     slot[r] = j < cnt ? (atomicAdd(&hist[pid], 1u) | (pid << 16)) : 0u;
   }
@@ -204,7 +236,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
     const uint32_t sl = r * kScatterThreads + tid;
     if (sl < cnt) {
       const uint32_t k = buf[sl];
-      const uint32_t p = part_of(k, a, slice_shift);
+      const uint32_t p = part_of(k, pf);
       dst[r] = gbase[p] + (sl - lbase[p]);
       keys_p[dst[r]] = k;
     }
@@ -356,19 +388,19 @@ static uint32_t fill_grid(uint64_t count) {
 static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 4u; }
 
 template <int BT>
-static hipError_t launch_scatter(const BbfPartLaunch& L, const AddrArgs& a) {
+static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
   constexpr uint32_t tile = BT * kScatterKpt;
   const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile);
   hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT>), lds);
   if (e != hipSuccess) return e;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
-  hipLaunchKernelGGL(k_part_scatter<BT>, dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, a, L.slice_shift,
-                     L.nparts, L.cursor, L.keys_p, L.pos_p);
+  hipLaunchKernelGGL(k_part_scatter<BT>, dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf, L.nparts,
+                     L.cursor, L.keys_p, L.pos_p);
   return hipGetLastError();
 }
 
 hipError_t partition_keys(const BbfPartLaunch& L) {
-  const AddrArgs a{L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post};
+  const PartFn pf = make_part_fn(L);
   const hipStream_t st = L.stream;
   const uint32_t nbins = L.nparts * kLanes;
   hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(nbins)), dim3(256), 0, st, L.counts,
@@ -381,38 +413,46 @@ hipError_t partition_keys(const BbfPartLaunch& L) {
   const uint32_t hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
   const uint32_t threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
   const uint32_t tile_log2 = (threads == 1024 ? 10u : threads == 512 ? 9u : 8u) + kScatterKptLog2;
-  hipLaunchKernelGGL(k_part_hist, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, a,
-                     L.slice_shift, L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
+  hipLaunchKernelGGL(k_part_hist, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, pf,
+                     L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.counts, nbins, L.offsets,
                      L.cursor);
   switch (threads) {
-    case 1024: return launch_scatter<1024>(L, a);
-    case 512: return launch_scatter<512>(L, a);
-    default: return launch_scatter<256>(L, a);
+    case 1024: return launch_scatter<1024>(L, pf);
+    case 512: return launch_scatter<512>(L, pf);
+    default: return launch_scatter<256>(L, pf);
   }
 }
 
-hipError_t part_reorder_matches(const BbfPartLaunch& L) {
+hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint64_t* d_base, uint64_t* d_count) {
+  hipError_t e = scan_tile_counts(L.ptile_cnt, L.ntiles, d_count, L.stream);
+  if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ntiles, L.pos_p, hp, d_base, L.stream);
+  return e;
+}
+
+hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits) {
   const hipStream_t st = L.stream;
-  hipError_t e = scan_tile_counts(L.ptile_cnt, L.ntiles, L.d_hits, st);
-  // hp (the matches' original positions) overwrites the grouped keys
-  if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ntiles, L.pos_p, L.keys_p, st);
-  if (e != hipSuccess) return e;
   const uint32_t G = L.hit_groups, nb = L.nbuckets;
-  e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_place), nb * 8u);
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_place), nb * 8u);
   if (e == hipSuccess) e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_bits), (1u << L.bucket_shift) / 8u);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, L.keys_p, L.d_hits, L.bucket_shift, nb,
-                     L.hcounts);
+  hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, hp, d_hits, L.bucket_shift, nb, L.hcounts);
   hipLaunchKernelGGL(k_hit_colscan, dim3((nb + kColB - 1) / kColB), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, nb, L.boffsets,
                      L.bcursor);
   // bucketed positions overwrite the grouped positions
-  hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, L.keys_p, L.d_hits, L.bucket_shift, nb,
+  hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, hp, d_hits, L.bucket_shift, nb,
                      L.hcounts, L.boffsets, L.pos_p);
   hipLaunchKernelGGL(k_hit_bits, dim3(nb), dim3(256), (1u << L.bucket_shift) / 8u, st, L.pos_p, L.boffsets,
                      L.bucket_shift, L.ntiles, L.bits, L.tile_cnt);
   return hipGetLastError();
+}
+
+hipError_t part_reorder_matches(const BbfPartLaunch& L) {
+  // hp (the matches' original positions) overwrites the grouped keys
+  hipError_t e = part_gather_matches(L, L.keys_p, nullptr, L.d_hits);
+  if (e == hipSuccess) e = part_sort_matches(L, L.keys_p, L.d_hits);
+  return e;
 }
 
 }  // namespace dtlh

@@ -77,7 +77,7 @@ class DtlError(RuntimeError):
 
 
 EXPORTS = [
-    "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe",
+    "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe", "cf_probe_ex",
     "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
     "dtl_bbf_config_count", "dtl_bbf_split_lanes", "dtl_version", "dtl_gather_bench",
@@ -108,6 +108,7 @@ def load(path=LIB_PATH):
     L.bbf_probe.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, vp]
     L.bbf_probe_ex.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
     L.cf_probe.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, vp]
+    L.cf_probe_ex.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
     L.cf_build_host.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, C.POINTER(C.c_uint64)]
     L.bbf_batch_plan_create.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), C.POINTER(vp)]
     L.bbf_batch_plan_run.argtypes = [vp, vp]
@@ -160,7 +161,7 @@ def load(path=LIB_PATH):
                  "dtl_mlp_get_params", "dtl_mlp_set_params"):
         getattr(L, name).restype = i32
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
-                 "cf_probe", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
+                 "cf_probe", "cf_probe_ex", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
                  "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_bbf_split_lanes",
                  "dtl_version",
                  "dtl_gather_bench"):
@@ -255,15 +256,21 @@ def cf_build_host(desc, keys_np, seed=0):
     return buf, ins.value
 
 
-def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None):
+def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None, params=None):
     import torch
     n = keys.numel()
     if out is None:
         out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
     if count is None:
         count = torch.zeros(1, dtype=torch.int64, device=keys.device)
-    _check("cf_probe", load().cf_probe(C.byref(desc), _ptr(buf), _ptr(keys), n, match_offset,
-                                       _ptr(out), _ptr(count), _stream(stream)))
+    L = load()
+    if params is None:
+        rc = L.cf_probe(C.byref(desc), _ptr(buf), _ptr(keys), n, match_offset, _ptr(out), _ptr(count),
+                        _stream(stream))
+    else:
+        rc = L.cf_probe_ex(C.byref(desc), _ptr(buf), _ptr(keys), n, match_offset, _ptr(out), _ptr(count),
+                           C.byref(params), _stream(stream))
+    _check("cf_probe", rc)
     return out, count
 
 

@@ -153,6 +153,17 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
 /* ---- cuckoo filter ----------------------------------------------------- */
 int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
              uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
+/* cf_probe with explicit parameters (NULL = defaults; same list either way).
+ * Tables of 64 MiB or more probed with at least 2^22 keys use the
+ * partitioned probe: keys grouped by the slice of their first bucket, that
+ * bucket and the victim cache tested from L2, then regrouped by the slice of
+ * their alternate bucket and that bucket tested; ~20 bytes of scratch per key.
+ * Honoured: flags DTL_PROBE_DIRECT / DTL_PROBE_PARTITIONED, slice_log2,
+ * block_threads (scatter workgroup size), ev_start / ev_stop.  No counterpart
+ * of its own in the reference (cf.cpp:49-54 has no launch parameters). */
+int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
+                uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, const dtl_probe_params* params,
+                dtl_stream_t stream);
 
 /* Host build: inserts n keys into a zeroed host buffer of buffer_bytes,
  * kicking victims with mt19937(seed) (the reference seeds a per-thread

@@ -260,7 +260,15 @@ def test_build_is_order_independent_and_idempotent():
 # ---------------------------------------------------------------------------
 # cuckoo filter
 # ---------------------------------------------------------------------------
-def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False):
+def cf_part_params(slice_log2=10):
+    p = D.ProbeParams(0, 0, 0, 0, D.PROBE_PARTITIONED)
+    p.slice_log2 = slice_log2
+    return p
+
+
+def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False, params=(None,)):
+    """The GPU list equals the oracle's for each probe parameter set given
+    (None = defaults)."""
     d = D.cf_desc(m, t, b)
     o = O.Cuckoo(m, t, b)
     rng = np.random.default_rng(seed)
@@ -274,11 +282,13 @@ def check_cf(m, t, b, load, n_probe, hit, seed, offset=0, overfill=False):
     assert ins > 0
     dbuf = torch.from_numpy(buf).to(dev())
     probe = mixed_keys(keys, n_probe, hit, seed + 7)
-    out, cnt = D.cf_probe(d, dbuf, to_dev(probe), offset)
-    torch.cuda.synchronize()
-    pos = D.positions(out, cnt)
     exp = o.batch_contains(buf, probe, offset)
-    assert np.array_equal(pos, exp)
+    kd = to_dev(probe)
+    for prm in params:
+        out, cnt = D.cf_probe(d, dbuf, kd, offset, params=prm)
+        torch.cuda.synchronize()
+        pos = D.positions(out, cnt)
+        assert np.array_equal(pos, exp), (t, b, None if prm is None else (prm.flags, prm.slice_log2))
     return d, buf, pos
 
 
@@ -290,19 +300,52 @@ def test_cuckoo_all_configs(t, b):
     # tables with one slot per bucket or 4-bit tags may overflow into the
     # victim cache; the probe must still match the oracle on the same bytes
     over = t == 4 or b == 1
-    check_cf((1 << 13) * bpb * 8, t, b, load, 50000, 0.5, seed=t * 10 + b, overfill=over)
-    check_cf(5003 * bpb * 8, t, b, load, 30000, 0.3, seed=t * 10 + b + 1, offset=99, overfill=over)
+    # direct kernel and the two-pass partitioned probe (slices of 1 KiB: tens of partitions)
+    prm = (D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT), cf_part_params(10))
+    check_cf((1 << 13) * bpb * 8, t, b, load, 50000, 0.5, seed=t * 10 + b, overfill=over, params=prm)
+    check_cf(5003 * bpb * 8, t, b, load, 30000, 0.3, seed=t * 10 + b + 1, offset=99, overfill=over, params=prm)
 
 
 def test_cuckoo_victim_cache_hits():
-    d, buf, pos = check_cf(64 * 64, 16, 4, 1.5, 20000, 0.5, seed=3, overfill=True)
+    d, buf, pos = check_cf(64 * 64, 16, 4, 1.5, 20000, 0.5, seed=3, overfill=True,
+                           params=(None, cf_part_params(7)))
     assert buf[d.victim_offset + 8] == 1
 
 
 def test_cuckoo_benchmark_config_small():
     """Config 3 shape at 1/64 scale: 16-bit tags, 4 slots, pow2, 95% load."""
-    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.01, seed=44)
-    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.5, seed=45)
+    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.01, seed=44, params=(None, cf_part_params(14)))
+    check_cf(1 << 26, 16, 4, 0.95, 1 << 20, 0.5, seed=45, params=(None, cf_part_params(14)))
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 2047, 2049, 4097, 100003])
+def test_cuckoo_partitioned_small_batches(n):
+    check_cf(3001 * 8 * 8, 16, 4, 0.9, n, 0.4, seed=n, offset=5, params=(cf_part_params(9),))
+
+
+def test_cuckoo_partitioned_large_batch():
+    """2^25 probe keys against a 32 MiB table (default-sized slices), both
+    passes and the hit sort at scale, against the oracle's threaded
+    restatement (pinned to the scalar oracle by tests/test_cpu_baseline.py)."""
+    import cpu_baseline as CB
+    m = (1 << 22) * 8 * 8
+    d = D.cf_desc(m, 16, 4)
+    o = O.Cuckoo(m, 16, 4)
+    rng = np.random.default_rng(8)
+    n = int(0.9 * d.bucket_cnt * 4)
+    keys = np.unique(rng.integers(0, 2**31, int(n * 1.2), dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(keys)
+    keys = keys[:n]
+    buf, ins = D.cf_build_host(d, keys, seed=8)
+    assert ins == len(keys)
+    probe = big_probe_keys(keys, (1 << 25) - 3, 9)
+    p = D.ProbeParams(0, 0, 0, 0, D.PROBE_PARTITIONED)
+    out, cnt = D.cf_probe(d, torch.from_numpy(buf).to(dev()), to_dev(probe), 3, params=p)
+    torch.cuda.synchronize()
+    got = D.positions(out, cnt).astype(np.uint64)
+    _, _, lists = CB.cf_probe(o, buf, probe, min(16, CB.physical_cores()))
+    exp = np.concatenate(lists).astype(np.uint64) + 3
+    assert len(got) == len(exp) and np.array_equal(got, exp)
 
 
 # ---------------------------------------------------------------------------
