@@ -53,19 +53,20 @@ constexpr int kPartScanThreads = 1024;
 
 // Partition function: partition = bucket >> slice_shift, where bucket is the
 // key's block (blocked Bloom) or first bucket (cuckoo: the same block_of of
-// h0), or (kind 1) the key's alternate cuckoo bucket.  The kind is uniform
-// over a launch.
+// h0), or (kind 1) the key's alternate cuckoo bucket.
 struct PartFn {
   AddrArgs a;
   uint32_t slice_shift;
-  uint32_t kind;
   uint32_t tag_mask;
   CfArgs cf;
 };
 
+// KIND is a template parameter of the kernels (the branch stays out of their
+// unrolled per-key loops; a runtime branch there cost the scatter ~10%).
+template <int KIND>
 __device__ __forceinline__ uint32_t part_of(uint32_t key, const PartFn& f) {
   uint32_t b = block_of(key * kMul.v[0], f.a);
-  if (f.kind) b = cf_alt(b, cf_tag(key, f.tag_mask), f.cf);
+  if constexpr (KIND == 1) b = cf_alt(b, cf_tag(key, f.tag_mask), f.cf);
   return b >> f.slice_shift;
 }
 
@@ -73,7 +74,6 @@ static PartFn make_part_fn(const BbfPartLaunch& L) {
   PartFn f;
   f.a = AddrArgs{L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post};
   f.slice_shift = L.slice_shift;
-  f.kind = L.part_kind;
   f.tag_mask = L.tag_mask;
   f.cf.magic = L.addr_magic;
   f.cf.mshift = L.addr_mshift;
@@ -109,6 +109,7 @@ __device__ __forceinline__ uint32_t lane_of(uint64_t i, uint32_t tile_log2) {
 // multiple of kHistThreads * kHistUnroll) into nparts * kLanes bins.  In LDS
 // the bins are lane-major (lane * nparts + partition): the keys of one wave
 // share a lane, so their atomics spread over all banks.
+template <int KIND>
 __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                             PartFn pf,
                                                             uint32_t nparts, uint32_t per_wg,
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __re
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
       const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
-      if (valid[u]) atomicAdd(&hist[lane_of(i, tile_log2) * nparts + part_of(key[u], pf)], 1u);
+      if (valid[u]) atomicAdd(&hist[lane_of(i, tile_log2) * nparts + part_of<KIND>(key[u], pf)], 1u);
     }
   }
   __syncthreads();
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
 // workgroups fit on a CU: one loads and sorts while the other writes (the
 // launch bound asks for 2 workgroups' waves per SIMD, <= 64 VGPRs at 1024
 // threads).
-template <int kScatterThreads>
+template <int kScatterThreads, int KIND>
 __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt)) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, PartFn pf, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
-    const uint32_t pid = part_of(key[r], pf);
+    const uint32_t pid = part_of<KIND>(key[r], pf);
     // rank in the partition (low 16 bits, tile < 2^16) and partition id
     slot[r] = j < cnt ? (atomicAdd(&hist[pid], 1u) | (pid << 16)) : 0u;
   }
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
     const uint32_t sl = r * kScatterThreads + tid;
     if (sl < cnt) {
       const uint32_t k = buf[sl];
-      const uint32_t p = part_of(k, pf);
+      const uint32_t p = part_of<KIND>(k, pf);
       dst[r] = gbase[p] + (sl - lbase[p]);
       keys_p[dst[r]] = k;
     }
@@ -387,20 +388,20 @@ static uint32_t fill_grid(uint64_t count) {
 
 static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 4u; }
 
-template <int BT>
+template <int BT, int KIND>
 static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
   constexpr uint32_t tile = BT * kScatterKpt;
   const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile);
-  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT>), lds);
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KIND>), lds);
   if (e != hipSuccess) return e;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
-  hipLaunchKernelGGL(k_part_scatter<BT>, dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf, L.nparts,
+  hipLaunchKernelGGL((k_part_scatter<BT, KIND>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf, L.nparts,
                      L.cursor, L.keys_p, L.pos_p);
   return hipGetLastError();
 }
 
-hipError_t partition_keys(const BbfPartLaunch& L) {
-  const PartFn pf = make_part_fn(L);
+template <int KIND>
+static hipError_t partition_keys_kind(const BbfPartLaunch& L, const PartFn& pf) {
   const hipStream_t st = L.stream;
   const uint32_t nbins = L.nparts * kLanes;
   hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(nbins)), dim3(256), 0, st, L.counts,
@@ -413,15 +414,20 @@ hipError_t partition_keys(const BbfPartLaunch& L) {
   const uint32_t hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
   const uint32_t threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
   const uint32_t tile_log2 = (threads == 1024 ? 10u : threads == 512 ? 9u : 8u) + kScatterKptLog2;
-  hipLaunchKernelGGL(k_part_hist, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, pf,
+  hipLaunchKernelGGL(k_part_hist<KIND>, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, pf,
                      L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.counts, nbins, L.offsets,
                      L.cursor);
   switch (threads) {
-    case 1024: return launch_scatter<1024>(L, pf);
-    case 512: return launch_scatter<512>(L, pf);
-    default: return launch_scatter<256>(L, pf);
+    case 1024: return launch_scatter<1024, KIND>(L, pf);
+    case 512: return launch_scatter<512, KIND>(L, pf);
+    default: return launch_scatter<256, KIND>(L, pf);
   }
+}
+
+hipError_t partition_keys(const BbfPartLaunch& L) {
+  const PartFn pf = make_part_fn(L);
+  return L.part_kind == 1 ? partition_keys_kind<1>(L, pf) : partition_keys_kind<0>(L, pf);
 }
 
 hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint64_t* d_base, uint64_t* d_count) {
