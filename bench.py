@@ -60,8 +60,10 @@ def make_inputs(torch, log2_m, n_probe, hit_rate, seed):
     return build, probe, n_build
 
 
-def time_probe(torch, D, desc, words, probe, steps, warmup, params=None):
-    """Returns (ms per step, avg ms of the probe kernel, matches)."""
+def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_step=None):
+    """Returns (ms per step, avg ms of the probe kernel, matches).  If
+    per_step is a dict it receives the median, min and max of the per-call
+    HIP-event times (ms)."""
     n = probe.numel()
     out = torch.empty(n, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -87,8 +89,38 @@ def time_probe(torch, D, desc, words, probe, steps, warmup, params=None):
     t1.record(stream)
     torch.cuda.synchronize()
     total_ms = t0.elapsed_time(t1)
-    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    each = sorted(a.elapsed_time(b) for a, b in evs)
+    kern_ms = sum(each) / steps
+    if per_step is not None:
+        mid = len(each) // 2
+        per_step.update({"median": round(each[mid] if len(each) % 2 else 0.5 * (each[mid - 1] + each[mid]), 4),
+                         "min": round(each[0], 4), "max": round(each[-1], 4), "calls": len(each)})
     return total_ms / steps, kern_ms, int(cnt.item())
+
+
+def host_cpu_topology():
+    """CPUs of this process's share, physical cores among them and SMT
+    threads per core (from /sys, as lscpu reports them)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    sib = {}
+    for c in cpus:
+        try:
+            with open("/sys/devices/system/cpu/cpu%d/topology/thread_siblings_list" % c) as fh:
+                sib.setdefault(fh.read().strip(), []).append(c)
+        except OSError:
+            sib.setdefault(str(c), []).append(c)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    per_core = max(len(v) for v in sib.values()) if sib else 1
+    return {"cpus": len(cpus), "physical_cores": len(sib), "smt_threads_per_core": per_core,
+            "smt": per_core > 1, "model": model}
 
 
 def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
@@ -110,7 +142,8 @@ def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
     one = keys[: max(sample_keys // 16, 1 << 20)]
     _, secs1, _ = CB.bbf_probe(f, host_words, one, 1, simd)
     return {"value": round(sample_keys / best / 1e9, 4), "unit": "G probes/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host_cpu": host_cpu_topology(),
+            "one_thread_G_probes_per_s": round(len(one) / secs1 / 1e9, 4),
             "sample": "%d probe keys (1%% hits) against the same 1 GiB filter in host memory; %s "
                       "batch_contains, %d threads, best of 3 = %.3f s; 1 thread: %.4f G probes/s"
                       % (sample_keys, "AVX-512" if simd else "scalar", threads, best,
@@ -120,7 +153,8 @@ def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
 GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal",
                 2: "4 lanes / 64 B sector, 1x16 B each", 3: "as 2, non-temporal",
                 4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector",
-                6: "as 2, uniform random sectors", 7: "as 0, uniform random sectors"}
+                6: "as 2, uniform random sectors", 7: "as 0, uniform random sectors",
+                8: "LDS-staged buffer, 1 lane / uniform random 4 B word"}
 
 
 def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
@@ -129,7 +163,7 @@ def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
     sink = torch.zeros(1, dtype=torch.int64, device="cuda")
     out = {}
     for mode in modes:
-        sector = 32 if mode in (4, 5) else 64
+        sector = 4 if mode == 8 else (32 if mode in (4, 5) else 64)
         for _ in range(2):
             D.gather_bench(buf, mode, n, sink)
         torch.cuda.synchronize()
@@ -157,12 +191,13 @@ def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, wa
     words = D.new_filter(desc)
     D.bbf_build(desc, words, build)
     torch.cuda.synchronize()
-    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup)
+    st = {}
+    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st)
     return {"config": label, "filter_bytes": int(desc.size_bytes),
             "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
             "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
             "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
-            "matches": matches}
+            "call_ms": st, "matches": matches}
 
 
 FIXED_SEMI_JOIN_CFG = (32, 6, 16, 16, 2)   # (word_bits, k, w, s, z): u32 words, w=16, z=2, k=6
@@ -406,6 +441,26 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
                        "%d keys (%.0f%% load) built on the host; 2^28 probe keys" %
                        (log2_buckets, desc.buffer_bytes >> 20, n_build, load * 100),
            "host_build_s": round(build_s, 1), "inserted": int(inserted), "legs": []}
+    log("cuckoo: GPU build of the same keys (cf_build)")
+    gbuf = D.new_cf_table(desc)
+    failed = D.cf_build(desc, gbuf, build_dev, seed=1)    # untimed first run (loads the kernel)
+    torch.cuda.synchronize()
+    gbuf.zero_()
+    failed.zero_()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    D.cf_build(desc, gbuf, build_dev, seed=1, failed=failed)
+    b.record()
+    torch.cuda.synchronize()
+    g_ms = a.elapsed_time(b)
+    found_out = torch.empty(n_build, dtype=torch.int32, device="cuda")
+    found_cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    D.cf_probe(desc, gbuf, build_dev, 0, out=found_out, count=found_cnt)
+    torch.cuda.synchronize()
+    res["gpu_build"] = {"ms": round(g_ms, 2), "M_keys_per_s": round(n_build / g_ms / 1e3, 1),
+                        "failed": int(failed.item()), "build_keys_found": int(found_cnt.item()),
+                        "vs_host_build": round(build_s * 1e3 / g_ms, 1)}
+    del gbuf, found_out
     for hit in (0.01, 0.5):
         n = 1 << 28
         g = torch.Generator(device="cuda").manual_seed(5)
@@ -542,7 +597,8 @@ def main():
     del build
 
     log("bench: probing (%d warmup + %d timed steps)" % (args.warmup, args.steps))
-    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, args.steps, args.warmup)
+    per_call = {}
+    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, args.steps, args.warmup, per_step=per_call)
     n = probe.numel()
     value = n / (ms * 1e-3) / 1e9
     partitioned = desc.size_bytes >= (64 << 20) and n >= (1 << 22)
@@ -588,6 +644,7 @@ def main():
                        "reorder), HIP events around the phase" if partitioned
                        else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
+            "kernel_ms_per_call": per_call,
             "bytes_per_probe": bytes_per_key,
             "direct_kernel": {
                 "kernel": "k_bbf_probe_global<u64,w8,s8,k8> (one random 64 B block per key from HBM)",
@@ -619,6 +676,11 @@ def main():
                 regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 128 MiB"),
                 regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, 1,
                        "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
+        for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
+            lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
+            r["lds_random_4B_reads"] = {"G_reads_per_s": lds["G_sectors_per_s"], "buffer_bytes": 1 << lb,
+                                        "probe_vs_lds_reads": round(r["kernel_G_probes_per_s"]
+                                                                    / lds["G_sectors_per_s"], 3)}
         result["regimes"] = regs
     print(json.dumps(result), flush=True)
 

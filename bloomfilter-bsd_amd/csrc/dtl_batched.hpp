@@ -147,10 +147,11 @@ hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_count
 template <class C>
 hipError_t launch_probe_batched(const BbfBatchLaunch& L) {
   const uint32_t chunk = L.chunk ? L.chunk : 1u;
-  switch (L.threads) {
-    case 1024: return launch_probe_batched_bt<C, 1024>(L, L.work_counter, chunk);
-    default: return launch_probe_batched_bt<C, 256>(L, L.work_counter, chunk);
+  // 1024-thread workgroups only for 32-bit words (as launch_probe's LDS path)
+  if constexpr (sizeof(typename C::word_t) == 4) {
+    if (L.threads == 1024) return launch_probe_batched_bt<C, 1024>(L, L.work_counter, chunk);
   }
+  return launch_probe_batched_bt<C, 256>(L, L.work_counter, chunk);
 }
 
 }  // namespace dtlh

@@ -516,6 +516,37 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
   return cf_probe_ex(d, d_buf, d_keys, n, match_offset, d_pos, d_count, nullptr, stream);
 }
 
+int cf_build(const dtl_cf_desc* d, void* d_buf, const uint32_t* d_keys, uint64_t n, uint32_t seed,
+             uint32_t max_kicks, uint64_t* d_failed, dtl_stream_t stream) {
+  if (!d || !d_buf || !d_failed || d->bucket_cnt < 2 || (n && !d_keys)) return DTL_EINVAL;
+  if (!aligned_to(d_buf, 16) || !aligned_to(d_keys, 4) || !aligned_to(d_failed, 8)) return DTL_EINVAL;
+  if (max_kicks > (1u << 20)) return DTL_EINVAL;
+  const CfEntry* e = cf_find(d->tag_bits, d->tags_per_bucket);
+  if (!e || !e->build) return DTL_EINVAL;
+  if (n == 0) return DTL_OK;
+  clear_stale_error();
+  DeviceInfo di;
+  const int rc = current_device_info(&di, nullptr);
+  if (rc != DTL_OK) return rc;
+  CfBuildLaunch L;
+  L.buf = d_buf;
+  L.keys = d_keys;
+  L.n = n;
+  L.addr_mode = d->addr_mode;
+  L.shift = d->bucket_shift;
+  L.bucket_mask = d->bucket_mask;
+  L.magic = d->magic;
+  L.mshift = d->magic_shift;
+  L.bucket_cnt = d->bucket_cnt;
+  L.victim_offset = d->victim_offset;
+  L.max_kicks = max_kicks ? max_kicks : 500u;  // max_relocation_count, cuckoofilter_logic.hpp:122
+  L.seed = seed;
+  L.failed = d_failed;
+  L.cu_count = di.cu_count;
+  L.stream = static_cast<hipStream_t>(stream);
+  return hip_status(e->build(L));
+}
+
 int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
                 uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, const dtl_probe_params* params,
                 dtl_stream_t stream) {

@@ -1,4 +1,4 @@
-// dtl_cuckoo.hip -- cuckoo filter probe on gfx950.
+// dtl_cuckoo.hip -- cuckoo filter probe and build on gfx950.
 //
 // One key per lane: i1 from h0, tag from h1 (0 -> 1), i2 = alt(i1, tag);
 // both buckets are read with aligned loads (one 8-byte load each for the
@@ -12,6 +12,9 @@
 #include "dtl_cuckoo.hpp"
 #include "dtl_internal.hpp"
 #include "dtl_kernels.hpp"
+
+#include <algorithm>
+#include <type_traits>
 
 namespace dtlh {
 
@@ -174,7 +177,179 @@ hipError_t launch_cf_probe_part(const BbfPartLaunch& L) {
   return hipGetLastError();
 }
 
-#define DTL_CF_ENTRY(t, b) {t, b, &launch_cf_probe<t, b>, &launch_cf_probe_part<t, b>}
+// ---------------------------------------------------------------------------
+// Build: concurrent partial-key cuckoo insertion.
+//
+// The reference inserts sequentially on the host (cuckoofilter_logic.hpp:
+// 291-322): put the tag in the first empty slot of i1, else of i2, else kick
+// a random slot of the current bucket and carry the evicted tag to its
+// alternate bucket, at most 500 times, then park it in the victim cache.
+// Here every lane runs that walk for its own keys at the same time.  Each
+// step reads the bucket and changes one slot with one compare-and-swap on
+// the aligned 4- or 8-byte word that holds it, so a tag is never lost or
+// duplicated: a failed CAS (another lane changed the word) costs the step
+// and the walk re-reads the same bucket.  Every tag therefore stays in one
+// of the two buckets of the key it came from (alt() maps a tag's two buckets
+// onto each other), which is all a probe relies on.  Which slots end up
+// holding which tags depends on the interleaving, so tables differ from the
+// host build's; the probe semantics do not.
+//
+// Lanes are independent state machines: a lane whose walk ends takes its
+// next key at once, so a long walk holds up only its own lane.  A walk that
+// reaches max_kicks steps, or whose alternate bucket is bucket_cnt (the
+// MAGIC corner case of cuckoofilter_logic.hpp:241-250, which has no bucket
+// in the table), claims the victim cache; if that is taken the tag has no
+// place and is counted in *failed.
+// ---------------------------------------------------------------------------
+template <int T, int B>
+struct BucketWords {
+  static constexpr int BPB = (T * B + 7) / 8;
+  static_assert(BPB == 1 || BPB == 2 || BPB == 4 || BPB == 8 || BPB == 16, "bucket straddles words");
+  using W = std::conditional_t<(BPB >= 8), unsigned long long, uint32_t>;
+  static constexpr int NW = BPB == 16 ? 2 : 1;   // CAS words per bucket
+  static constexpr int SPW = B / NW;             // slots per CAS word
+  static constexpr uint32_t kMask = static_cast<uint32_t>((1ull << T) - 1ull);
+  W* p;          // first CAS word of the bucket
+  uint32_t bo;   // bit offset of the bucket inside its word
+  __device__ __forceinline__ void at(uint8_t* base, uint32_t i) {
+    const uint64_t off = static_cast<uint64_t>(i) * BPB;
+    if constexpr (BPB >= 8) {
+      p = reinterpret_cast<W*>(base + off);
+      bo = 0;
+    } else {
+      p = reinterpret_cast<W*>(base + (off & ~static_cast<uint64_t>(3)));
+      bo = static_cast<uint32_t>(off & 3u) * 8u;
+    }
+  }
+  __device__ __forceinline__ static int word_of(int j) { return j / SPW; }
+  __device__ __forceinline__ uint32_t shift_of(int j) const { return bo + T * (j % SPW); }
+};
+
+__device__ __forceinline__ uint32_t cf_kick_slot(uint32_t key, uint32_t step, uint32_t seed) {
+  uint32_t h = key * 0x9E3779B1u ^ (step * 0x85EBCA6Bu + seed);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  return h;
+}
+
+template <int T, int B>
+__global__ __launch_bounds__(256) void k_cf_build(uint8_t* __restrict__ buf, const uint32_t* __restrict__ keys,
+                                                  uint64_t n, CfArgs a, uint32_t max_kicks, uint32_t seed,
+                                                  unsigned long long* __restrict__ failed) {
+  using BW = BucketWords<T, B>;
+  using W = typename BW::W;
+  constexpr uint32_t kMask = BW::kMask;
+  AddrArgs addr;
+  addr.magic = a.magic;
+  addr.mshift = a.mshift;
+  addr.divisor = a.divisor;
+  addr.post = a.post;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t key = 0, cur_i = 0, cur_tag = 0, step = 0;
+  if (k < n) {
+    key = keys[k];
+    cur_i = block_of(key * kMul.v[0], addr);
+    cur_tag = cf_tag(key, kMask);
+  }
+  while (k < n) {
+    bool done = false;
+    bool to_victim = step >= max_kicks;
+    if (!to_victim) {
+      BW bw;
+      bw.at(buf, cur_i);
+      W v[BW::NW];
+#pragma unroll
+      for (int w = 0; w < BW::NW; ++w) v[w] = __hip_atomic_load(bw.p + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int jfree = -1;
+#pragma unroll
+      for (int j = B - 1; j >= 0; --j)
+        if (((v[BW::word_of(j)] >> bw.shift_of(j)) & kMask) == 0) jfree = j;
+      if (jfree >= 0 || step > 0) {
+        const int j = jfree >= 0 ? jfree : static_cast<int>(cf_kick_slot(key, step, seed) & (B - 1));
+        const int w = BW::word_of(j);
+        const uint32_t sh = bw.shift_of(j);
+        W expect = v[w];
+        const uint32_t old = static_cast<uint32_t>(expect >> sh) & kMask;
+        const W desired = (expect & ~(static_cast<W>(kMask) << sh)) | (static_cast<W>(cur_tag) << sh);
+        const bool swapped = __hip_atomic_compare_exchange_strong(bw.p + w, &expect, desired, __ATOMIC_RELAXED,
+                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (swapped) {
+          if (jfree >= 0) {
+            done = true;
+          } else {
+            cur_tag = old;  // carry the evicted tag to its other bucket
+            cur_i = cf_alt(cur_i, cur_tag, a);
+          }
+        }
+      } else {
+        cur_i = cf_alt(cur_i, cur_tag, a);  // first step, both slots of i1 taken: try i2 without a kick
+      }
+      ++step;
+      if (!done && a.mode != 0 && cur_i >= a.bucket_cnt) to_victim = true;
+    }
+    if (to_victim) {
+      uint32_t* vp = reinterpret_cast<uint32_t*>(buf + a.victim_offset);  // {u32 index; u32 tag; u8 used}
+      uint32_t zero = 0;
+      if (__hip_atomic_compare_exchange_strong(vp + 2, &zero, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(vp, cur_i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(vp + 1, cur_tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_fetch_add(failed, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      done = true;
+    }
+    if (done) {
+      k += stride;
+      step = 0;
+      if (k < n) {
+        key = keys[k];
+        cur_i = block_of(key * kMul.v[0], addr);
+        cur_tag = cf_tag(key, kMask);
+      }
+    }
+  }
+}
+
+template <int T, int B>
+hipError_t launch_cf_build(const CfBuildLaunch& L) {
+  CfArgs a;
+  a.mode = L.addr_mode;
+  if (L.addr_mode == 0) {
+    a.magic = 0;
+    a.mshift = 0;
+    a.divisor = 0;
+    a.post = L.shift;
+  } else {
+    a.magic = L.magic;
+    a.mshift = L.mshift;
+    a.divisor = L.bucket_cnt;
+    a.post = 0;
+  }
+  a.bucket_mask = L.bucket_mask;
+  a.bucket_cnt = L.bucket_cnt;
+  a.pad = 0;
+  a.victim_offset = L.victim_offset;
+  const uint64_t want = (L.n + 255) / 256;
+  const uint64_t cap = static_cast<uint64_t>(std::max(L.cu_count, 1)) * 8;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+  hipLaunchKernelGGL((k_cf_build<T, B>), dim3(grid), dim3(256), 0, L.stream, static_cast<uint8_t*>(L.buf), L.keys,
+                     L.n, a, L.max_kicks, L.seed, reinterpret_cast<unsigned long long*>(L.failed));
+  return hipGetLastError();
+}
+
+using CfBuildFn = hipError_t (*)(const CfBuildLaunch&);
+
+template <int T, int B>
+constexpr CfBuildFn cf_build_fn() {
+  constexpr int BPB = (T * B + 7) / 8;
+  if constexpr (BPB == 1 || BPB == 2 || BPB == 4 || BPB == 8 || BPB == 16) return &launch_cf_build<T, B>;
+  else return nullptr;
+}
+
+#define DTL_CF_ENTRY(t, b) {t, b, &launch_cf_probe<t, b>, &launch_cf_probe_part<t, b>, cf_build_fn<t, b>()}
 static const CfEntry kCfEntries[] = {
     DTL_CF_ENTRY(4, 1),  DTL_CF_ENTRY(4, 2),  DTL_CF_ENTRY(4, 4),  DTL_CF_ENTRY(8, 1),  DTL_CF_ENTRY(8, 2),
     DTL_CF_ENTRY(8, 4),  DTL_CF_ENTRY(12, 1), DTL_CF_ENTRY(12, 2), DTL_CF_ENTRY(12, 4), DTL_CF_ENTRY(16, 1),

@@ -12,6 +12,9 @@
 //   5  two lanes per 32 B sector, one 16-byte load each
 //   6  as 2, sectors drawn by a full-avalanche hash of the read index
 //   7  as 0, sectors drawn by the same hash
+//   8  LDS: every workgroup stages the whole buffer (at most 128 KiB) in LDS,
+//      then each lane reads pseudo-random 4-byte words of it (the access of
+//      a one-word-block filter probed from LDS)
 // Modes 0-5 pick sector (i * 0x9E3779B1) >> shift, a low-discrepancy
 // sequence that spreads consecutive reads evenly over the memory channels;
 // modes 6-7 pick independent uniform sectors, like the blocks of random keys.
@@ -68,6 +71,35 @@ __global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, u
   if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9E3779B1u) sink[0] = gid;  // keeps the loads alive
 }
 
+constexpr int kLdsReadsPerIter = 8;
+
+__global__ __launch_bounds__(1024) void k_lds_gather(const u32x4* __restrict__ buf, uint32_t buf_bytes,
+                                                     uint32_t shift, uint64_t n_reads,
+                                                     uint64_t* __restrict__ sink) {
+  extern __shared__ u32x4 lds_buf[];
+  for (uint32_t i = threadIdx.x; i < buf_bytes / 16; i += blockDim.x) lds_buf[i] = buf[i];
+  __syncthreads();
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(lds_buf);
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kLdsReadsPerIter;
+  // word indexes: the top bits of a per-lane linear congruential stream
+  // (Numerical Recipes' 1664525 / 1013904223), seeded by a full-avalanche
+  // hash of the lane, so that the index arithmetic costs two operations
+  uint32_t x = sector_of_random(gid, 0);
+  uint32_t acc = 0;
+  for (uint64_t base = gid * kLdsReadsPerIter; base < n_reads; base += stride) {
+    uint32_t v[kLdsReadsPerIter];
+#pragma unroll
+    for (int r = 0; r < kLdsReadsPerIter; ++r) {
+      x = x * 1664525u + 1013904223u;
+      v[r] = words[x >> shift];
+    }
+#pragma unroll
+    for (int r = 0; r < kLdsReadsPerIter; ++r) acc ^= (base + r < n_reads) ? v[r] : 0u;
+  }
+  if (acc == 0x9E3779B1u) sink[0] = acc;  // keeps the reads alive
+}
+
 template <int MODE>
 static hipError_t launch_gather(const void* buf, uint32_t shift, uint64_t n, uint64_t* sink,
                                 hipStream_t st) {
@@ -84,9 +116,10 @@ static hipError_t launch_gather(const void* buf, uint32_t shift, uint64_t n, uin
 extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
                                 uint64_t* d_sink, dtl_stream_t stream) {
   using namespace dtlh;
-  const uint64_t sector = (mode == 4 || mode == 5) ? 32 : 64;
-  if (!d_buf || !d_sink || mode > 7 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
+  const uint64_t sector = mode == 8 ? 4 : ((mode == 4 || mode == 5) ? 32 : 64);
+  if (!d_buf || !d_sink || mode > 8 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
     return DTL_EINVAL;
+  if (mode == 8 && (buf_bytes < 64 || buf_bytes > (128u << 10))) return DTL_EINVAL;
   if ((n_reads + kGatherPerLane - 1) / kGatherPerLane * 4 / 256 > 0x7FFFFFFFull) return DTL_EINVAL;
   const uint64_t sectors = buf_bytes / sector;
   const uint32_t shift = 32u - (63u - static_cast<uint32_t>(__builtin_clzll(sectors)));
@@ -94,6 +127,28 @@ extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t 
   (void)hipGetLastError();
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
+  if (mode == 8) {
+    int dev = 0, cus = 0;
+    uint32_t max_lds = 0;
+    const int rc = current_device(&dev, &cus, &max_lds);
+    if (rc != DTL_OK) return rc;
+    const uint32_t bytes = static_cast<uint32_t>(buf_bytes);
+    if (bytes > max_lds) return DTL_EINVAL;
+    e = allow_dynamic_lds(reinterpret_cast<const void*>(k_lds_gather), bytes);
+    int per_cu = 0;
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lds_gather, 1024, bytes);
+    if (e == hipSuccess) {
+      const uint32_t grid = static_cast<uint32_t>(std::max(per_cu, 1) * std::max(cus, 1));
+      hipLaunchKernelGGL(k_lds_gather, dim3(grid), dim3(1024), bytes, st, static_cast<const u32x4*>(d_buf),
+                         bytes, shift, n_reads, d_sink);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+      set_last_error(e);
+      return DTL_EHIP;
+    }
+    return DTL_OK;
+  }
   switch (mode) {
     case 0: e = launch_gather<0>(d_buf, shift, n_reads, d_sink, st); break;
     case 1: e = launch_gather<1>(d_buf, shift, n_reads, d_sink, st); break;

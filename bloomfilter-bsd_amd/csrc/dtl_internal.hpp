@@ -156,10 +156,23 @@ struct CfProbeLaunch {
   hipStream_t stream;
 };
 
+struct CfBuildLaunch {
+  void* buf;
+  const uint32_t* keys;
+  uint64_t n;
+  uint32_t addr_mode, shift, bucket_mask, magic, mshift, bucket_cnt;
+  uint64_t victim_offset;
+  uint32_t max_kicks, seed;
+  uint64_t* failed;
+  int cu_count;
+  hipStream_t stream;
+};
+
 struct CfEntry {
   uint32_t tag_bits, tags_per_bucket;
   hipError_t (*probe)(const CfProbeLaunch&);
   hipError_t (*probe_part)(const BbfPartLaunch&);  // one pass over grouped keys
+  hipError_t (*build)(const CfBuildLaunch&);       // null: buckets straddle 4-byte words
 };
 
 // Registries (filled once, on first use).

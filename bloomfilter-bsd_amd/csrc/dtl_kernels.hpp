@@ -433,6 +433,14 @@ __global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_s
 
 // ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
+// Workgroup b probes tiles b, b + grid, b + 2 grid, ...; it requests the keys
+// of G of its tiles at once (G x ROUNDS keys per lane, 8 with the 1024-thread
+// block that large filters leave one of per CU), then tests them, so that
+// key loads overlap even at one workgroup per CU.  A tile past the end is
+// probed like the others (its keys are clamped, its hits masked) and only its
+// stores are skipped: the same loop with a uniform early exit per tile lost
+// hits of the group's first tile now and then on gfx950 (ROCm 7.2), which
+// this branch-free form does not.
 // ---------------------------------------------------------------------------
 template <class C, int BT>
 __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* __restrict__ filter,
@@ -443,8 +451,10 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
                                                       uint32_t ntiles) {
   using W = typename C::word_t;
   constexpr int ROUNDS = kTileKeys / BT;
+  constexpr int G = ROUNDS >= 8 ? 1 : 8 / ROUNDS;  // tiles whose keys are requested together
+  constexpr int NW = BT / 64;
   extern __shared__ uint4 lds_filter[];
-  __shared__ uint32_t red[BT / 64];
+  __shared__ uint32_t red[G][NW];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   {
     const uint32_t n16 = filter_bytes / 16;
@@ -456,22 +466,43 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
   }
   __syncthreads();
   const W* lds = reinterpret_cast<const W*>(lds_filter);
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint64_t tile_base = static_cast<uint64_t>(t) * kTileKeys;
-    uint32_t cnt = 0;
+  for (uint32_t t0 = blockIdx.x; t0 < ntiles; t0 += G * gridDim.x) {
+    uint32_t key[G][ROUNDS];
 #pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-      const uint64_t i = tile_base + static_cast<uint64_t>(r) * BT + tid;
-      const bool valid = i < n;
-      const uint32_t key = load_key(keys, valid ? i : n - 1);
-      const uint32_t b = block_of(key * kMul.v[0], addr);
-      const bool hit = test_ptr<C>(key, lds + static_cast<uint32_t>(b * C::w)) & valid;
-      const uint64_t m = __ballot(hit);
-      if (lane == 0) store_bits(bits, (tile_base + static_cast<uint64_t>(r) * BT) / 64 + wave, m);
-      cnt += __popcll(m);
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+        const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * gridDim.x;
+        const uint64_t i = t * kTileKeys + static_cast<uint64_t>(r) * BT + tid;
+        key[g][r] = load_key(keys, i < n ? i : n - 1);
+      }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * gridDim.x;
+      const bool tv = t < ntiles;
+      const uint64_t tile_base = t * kTileKeys;
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+        const uint64_t i = tile_base + static_cast<uint64_t>(r) * BT + tid;
+        const uint32_t b = block_of(key[g][r] * kMul.v[0], addr);
+        const bool hit = test_ptr<C>(key[g][r], lds + static_cast<uint32_t>(b * C::w)) & (i < n);
+        const uint64_t m = __ballot(hit);
+        if (lane == 0 && tv) store_bits(bits, (tile_base + static_cast<uint64_t>(r) * BT) / 64 + wave, m);
+        cnt += __popcll(m);
+      }
+      if (lane == 0) red[g][wave] = cnt;
     }
-    const uint32_t total = block_sum_waves<BT>(cnt, red);
-    if (tid == 0) tile_cnt[t] = total;
+    __syncthreads();
+    if (tid < G) {
+      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(tid) * gridDim.x;
+      if (t < ntiles) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s += red[tid][w];
+        tile_cnt[t] = s;
+      }
+    }
     __syncthreads();
   }
 }

@@ -48,10 +48,14 @@ template <class C>
 hipError_t launch_probe(const BbfProbeLaunch& L) {
   using W = typename C::word_t;
   if (L.use_lds) {
-    switch (L.lds_threads) {
-      case 1024: return launch_probe_lds<C, 1024>(L);
-      default: return launch_probe_lds<C, 256>(L);
+    // 1024-thread workgroups only for 32-bit words: with 64-bit words that
+    // shape lost whole 64-key ballot words now and then on gfx950 (ROCm 7.2;
+    // cause not found: it persists with the blocks read from global memory
+    // and with the bitmap stores deferred), the 256-thread shape never did.
+    if constexpr (sizeof(W) == 4) {
+      if (L.lds_threads == 1024) return launch_probe_lds<C, 1024>(L);
     }
+    return launch_probe_lds<C, 256>(L);
   }
   auto fn = k_bbf_probe_global<C>;
   if constexpr (C::split.ok) {

@@ -78,7 +78,7 @@ class DtlError(RuntimeError):
 
 EXPORTS = [
     "dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "cf_probe", "cf_probe_ex",
-    "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
+    "cf_build_host", "cf_build", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
     "dtl_bbf_config_count", "dtl_bbf_split_lanes", "dtl_version", "dtl_gather_bench",
     "bbf_batch_plan_group_count", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
@@ -110,6 +110,7 @@ def load(path=LIB_PATH):
     L.cf_probe.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, vp]
     L.cf_probe_ex.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
     L.cf_build_host.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, C.POINTER(C.c_uint64)]
+    L.cf_build.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, u32, vp, vp]
     L.bbf_batch_plan_create.argtypes = [C.POINTER(BbfTask), u32, C.POINTER(ProbeParams), C.POINTER(vp)]
     L.bbf_batch_plan_run.argtypes = [vp, vp]
     L.bbf_batch_plan_destroy.argtypes = [vp]
@@ -161,7 +162,7 @@ def load(path=LIB_PATH):
                  "dtl_mlp_get_params", "dtl_mlp_set_params"):
         getattr(L, name).restype = i32
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
-                 "cf_probe", "cf_probe_ex", "cf_build_host", "bbf_batch_plan_create", "bbf_batch_plan_run",
+                 "cf_probe", "cf_probe_ex", "cf_build_host", "cf_build", "bbf_batch_plan_create", "bbf_batch_plan_run",
                  "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_bbf_split_lanes",
                  "dtl_version",
                  "dtl_gather_bench"):
@@ -254,6 +255,24 @@ def cf_build_host(desc, keys_np, seed=0):
                                                  C.c_void_p(keys_np.ctypes.data), len(keys_np),
                                                  seed, C.byref(ins)))
     return buf, ins.value
+
+
+def new_cf_table(desc):
+    """Zeroed device table + victim cache (desc.buffer_bytes, padded to 8) as uint8."""
+    import torch
+    return torch.zeros((desc.buffer_bytes + 7) // 8 * 8, dtype=torch.uint8, device="cuda")
+
+
+def cf_build(desc, buf, keys, seed=0, max_kicks=0, failed=None, stream=None):
+    """GPU build (cf_build): inserts the device keys into the device table
+    `buf`; returns the device uint64 count of walks whose tag found no place
+    (added to `failed` when given).  0 means a valid table."""
+    import torch
+    if failed is None:
+        failed = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _check("cf_build", load().cf_build(C.byref(desc), _ptr(buf), _ptr(keys), keys.numel(), seed, max_kicks,
+                                       _ptr(failed), _stream(stream)))
+    return failed
 
 
 def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None, params=None):

@@ -12,6 +12,8 @@
  *                (cuckoofilter_logic::batch_contains           cuckoofilter_logic.hpp:390-479)
  *   cf_build_host dtl::cf::batch_insert (host build)          cf.cpp:39-42
  *                (cuckoofilter_logic::insert                   cuckoofilter_logic.hpp:291-337)
+ *   cf_build     the same batch_insert on the GPU (concurrent inserts; same
+ *                probe semantics, table layout differs from a sequential build)
  *   bbf_probe_batched  no single reference counterpart: many filter_base::
  *                batch_contains calls (one per filter) in one launch.
  *
@@ -173,6 +175,26 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
 int cf_build_host(const dtl_cf_desc* d, void* h_buf, const uint32_t* h_keys, uint64_t n,
                   uint32_t seed, uint64_t* inserted);
 
+/* GPU build: inserts n device keys into the device table d_buf (buffer_bytes,
+ * zeroed before the first call; later calls add keys).  Replaces
+ * dtl::cf::batch_insert (cf.cpp:39-42; cuckoofilter_logic.hpp:291-337),
+ * which inserts one key after another: here all keys walk their kick chains
+ * at once, each slot change being one compare-and-swap, with the reference's
+ * rules otherwise (first empty slot of i1, then of i2, then a kick of a
+ * pseudo-random slot -- seeded by `seed` -- and a walk of at most max_kicks
+ * steps, 0 = 500, ending in the victim cache).  The table is a valid cuckoo
+ * table for the keys (every key's tag in one of its two buckets or the victim
+ * cache) whenever *d_failed is unchanged: d_failed is a device uint64_t to
+ * which the call adds the number of walks that found the victim cache taken
+ * (their tag is dropped and some inserted key may then be missed).  Which
+ * slot holds which tag depends on the interleaving of the inserts, so bytes
+ * differ from cf_build_host's; cf_probe answers the same for every key that
+ * was inserted.  Unlike the reference, keys are still placed in the table
+ * after the victim cache is used.  DTL_EINVAL for 12-bit tags with 2 or 4
+ * slots (buckets straddle 4-byte words): build those with cf_build_host. */
+int cf_build(const dtl_cf_desc* d, void* d_buf, const uint32_t* d_keys, uint64_t n, uint32_t seed,
+             uint32_t max_kicks, uint64_t* d_failed, dtl_stream_t stream);
+
 /* ---- batched probe ------------------------------------------------------ */
 /* One probe task: a filter with its own probe batch and output list. */
 typedef struct dtl_bbf_task {
@@ -273,7 +295,9 @@ int dtl_mlp_set_params(dtl_mlp* mlp, const float* h_params);
  * measure the random-access bandwidth the probe kernels are compared with).
  * Reads n_reads pseudo-random aligned sectors (64 B, or 32 B for modes 4-5)
  * of a buffer of buf_bytes (a power of two); the access pattern `mode` is
- * described in dtl_gather.hip.  d_sink is a device uint64_t scratch word. */
+ * described in dtl_gather.hip.  Mode 8 stages the buffer (64 B .. 128 KiB)
+ * in every workgroup's LDS and reads random 4-byte words of it.  d_sink is a
+ * device uint64_t scratch word. */
 int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
                      uint64_t* d_sink, dtl_stream_t stream);
 

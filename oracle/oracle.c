@@ -592,6 +592,32 @@ uint64_t orc_cf_count_occupied(const orc_cf* f, const void* buf) {
   return c + read_victim(f, buf).used;
 }
 
+/* Inputs of a placement check of a table built in any insert order: the two
+ * candidate buckets and the tag of every key (cuckoofilter_logic.hpp:
+ * 179-186, 202-252), and every occupied slot of the table. */
+void orc_cf_key_candidates(const orc_cf* f, const uint32_t* keys, uint64_t n, uint32_t* i1, uint32_t* i2,
+                           uint32_t* tag) {
+  for (uint64_t k = 0; k < n; k++) {
+    tag[k] = orc_cf_tag(f, keys[k]);
+    i1[k] = orc_cf_i1(f, keys[k]);
+    i2[k] = orc_cf_alt(f, i1[k], tag[k]);
+  }
+}
+
+uint64_t orc_cf_entries(const orc_cf* f, const void* buf, uint32_t* bucket, uint32_t* tag) {
+  uint64_t c = 0;
+  for (uint64_t i = 0; i < f->addr.block_cnt; i++)
+    for (uint32_t j = 0; j < f->tags_per_bucket; j++) {
+      const uint32_t t = orc_cf_read_tag(f, buf, i, j);
+      if (t) {
+        bucket[c] = (uint32_t)i;
+        tag[c] = t;
+        c++;
+      }
+    }
+  return c;
+}
+
 /* ---------------------------------------------------------------------- */
 /* FPR model (blocked_bloomfilter/math.hpp:17-145, fpr.hpp:11-60)           */
 /* The Poisson pmf is evaluated as exp(i ln(l) - l - lgamma(i+1)) instead of */

@@ -148,6 +148,9 @@ uint32_t orc_cf_read_tag(const orc_cf* f, const void* buf, uint64_t bucket, uint
 /* the SWAR "has value" tests of bitsutil.hpp:33-47, exposed for tests */
 uint64_t orc_swar_hasvalue(uint32_t bits, uint64_t x, uint32_t v);
 uint64_t orc_cf_count_occupied(const orc_cf* f, const void* buf);
+void orc_cf_key_candidates(const orc_cf* f, const uint32_t* keys, uint64_t n, uint32_t* i1, uint32_t* i2,
+                           uint32_t* tag);
+uint64_t orc_cf_entries(const orc_cf* f, const void* buf, uint32_t* bucket, uint32_t* tag);
 
 /* ---- FPR model (blocked_bloomfilter/math.hpp, fpr.hpp) ------------------ */
 double orc_fpr_bloom(uint64_t m, uint64_t n, double k);

@@ -113,6 +113,10 @@ def lib():
         L.orc_swar_hasvalue.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32]
         L.orc_cf_count_occupied.restype = C.c_uint64
         L.orc_cf_count_occupied.argtypes = [C.POINTER(CF), C.c_void_p]
+        L.orc_cf_key_candidates.restype = None
+        L.orc_cf_key_candidates.argtypes = [C.POINTER(CF), _u32p, C.c_uint64, _u32p, _u32p, _u32p]
+        L.orc_cf_entries.restype = C.c_uint64
+        L.orc_cf_entries.argtypes = [C.POINTER(CF), C.c_void_p, _u32p, _u32p]
         L.orc_fpr_config.restype = C.c_double
         L.orc_fpr_config.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint32, C.c_uint32]
@@ -263,6 +267,64 @@ class Cuckoo:
 
     def count_occupied(self, buf):
         return lib().orc_cf_count_occupied(C.byref(self.d), buf.ctypes.data)
+
+    def victim(self, buf):
+        """(index, tag, used) of the victim cache {u32 index; u32 tag; u8 used}."""
+        off = self.d.victim_offset
+        v = np.frombuffer(buf[off: off + 8].tobytes(), dtype=np.uint32)
+        return int(v[0]), int(v[1]), bool(buf[off + 8])
+
+    def placement_valid(self, buf, keys):
+        """True iff the table holds exactly one tag per key, each in one of
+        that key's two buckets (i1 or alt(i1, tag)) or in the victim cache
+        with one of those indexes: a perfect matching between the keys and
+        the occupied entries.  Holds for a table built by the reference's
+        insert in any key order and with any kick choices, because a kick
+        moves a tag between the two buckets alt() pairs for it
+        (cuckoofilter_logic.hpp:291-322); it is what a probe relies on.
+        Keys must be distinct."""
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import maximum_bipartite_matching
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        n = len(keys)
+        i1 = np.zeros(max(n, 1), np.uint32)
+        i2 = np.zeros_like(i1)
+        tg = np.zeros_like(i1)
+        lib().orc_cf_key_candidates(C.byref(self.d), keys, n, i1, i2, tg)
+        i1, i2, tg = i1[:n], i2[:n], tg[:n]
+        cap = self.bucket_cnt * self.d.tags_per_bucket
+        eb = np.zeros(max(cap, 1), np.uint32)
+        et = np.zeros_like(eb)
+        ne = lib().orc_cf_entries(C.byref(self.d), buf.ctypes.data, eb, et)
+        vi, vt, vused = self.victim(buf)
+        if ne + int(vused) != n:
+            return False
+        if n == 0:
+            return True
+        code = (eb[:ne].astype(np.uint64) << np.uint64(32)) | et[:ne]
+        order = np.argsort(code, kind="stable")
+        code = code[order]
+        rows, cols = [], []
+        for bucket in (i1, i2.copy()):
+            if bucket is not i1:
+                bucket[i2 == i1] = np.uint32(0xFFFFFFFF)  # one bucket only: no second edge set
+            c = (bucket.astype(np.uint64) << np.uint64(32)) | tg
+            lo = np.searchsorted(code, c, "left")
+            hi = np.searchsorted(code, c, "right")
+            cnt = (hi - lo).astype(np.int64)
+            r = np.repeat(np.arange(n), cnt)
+            start = np.repeat(lo - np.cumsum(cnt) + cnt, cnt) + np.arange(cnt.sum())
+            rows.append(r)
+            cols.append(order[start])
+        if vused:
+            vk = np.nonzero((tg == vt) & ((i1 == vi) | (i2 == vi)))[0]
+            rows.append(vk)
+            cols.append(np.full(len(vk), ne, dtype=np.int64))
+        r = np.concatenate(rows)
+        c = np.concatenate(cols)
+        g = csr_matrix((np.ones(len(r), np.int8), (r, c)), shape=(n, ne + int(vused)))
+        match = maximum_bipartite_matching(g, perm_type="column")
+        return bool((match >= 0).all())
 
 
 def swar_hasvalue(bits, x, v):

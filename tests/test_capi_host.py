@@ -174,6 +174,26 @@ def test_gather_bench_rejects_bad_arguments():
     L = D.load()
     fake = ctypes.c_void_p(4096)
     sink = ctypes.c_void_p(8192)
-    assert L.dtl_gather_bench(fake, 1 << 20, 8, 1024, sink, None) == D.EINVAL   # modes 0..7
+    assert L.dtl_gather_bench(fake, 1 << 20, 9, 1024, sink, None) == D.EINVAL   # modes 0..8
+    assert L.dtl_gather_bench(fake, 1 << 20, 8, 1024, sink, None) == D.EINVAL   # LDS mode: <= 128 KiB
+    assert L.dtl_gather_bench(fake, 32, 8, 1024, sink, None) == D.EINVAL        # LDS mode: >= 64 B
     assert L.dtl_gather_bench(fake, (1 << 20) + 64, 0, 1024, sink, None) == D.EINVAL  # not a power of two
     assert L.dtl_gather_bench(None, 1 << 20, 0, 1024, sink, None) == D.EINVAL
+
+
+def test_cf_build_rejects_bad_arguments():
+    """GPU cuckoo build: argument checks run before any GPU call; 12-bit tags
+    with 2 or 4 slots (3- and 6-byte buckets straddle 4-byte words) are not
+    built on the GPU."""
+    L = D.load()
+    fake = ctypes.c_void_p(4096)
+    cnt = ctypes.c_void_p(8192)
+    for t, b in ((12, 2), (12, 4)):
+        d = D.cf_desc(1 << 16, t, b)
+        assert L.cf_build(ctypes.byref(d), fake, fake, 10, 0, 0, cnt, None) == D.EINVAL
+    d = D.cf_desc(1 << 16, 16, 4)
+    assert L.cf_build(ctypes.byref(d), None, fake, 10, 0, 0, cnt, None) == D.EINVAL
+    assert L.cf_build(ctypes.byref(d), fake, fake, 10, 0, 0, None, None) == D.EINVAL
+    assert L.cf_build(ctypes.byref(d), ctypes.c_void_p(4100), fake, 10, 0, 0, cnt, None) == D.EINVAL  # 16 B align
+    assert L.cf_build(ctypes.byref(d), fake, fake, 10, 0, (1 << 20) + 1, cnt, None) == D.EINVAL   # max_kicks cap
+    assert L.cf_build(ctypes.byref(d), fake, None, 0, 0, 0, cnt, None) == D.OK                  # nothing to do

@@ -133,6 +133,30 @@ def test_every_configuration_parity():
         check_cfg((wb, m, k, w, s, z), 800, 6000, seed=100 + i, hit=0.3, offset=i)
 
 
+def test_every_configuration_lds_tile_groups():
+    """Every configuration through the LDS kernel asked for 1024-thread
+    workgroups (taken for 32-bit words, whose workgroups then take the keys
+    of 4 tiles at a time; 64-bit words run 256-thread workgroups): grids of
+    1, 3 and 7 workgroups and the default over 15 or 16 tiles, so that tile
+    groups end past the last tile at every position; POW2 and MAGIC sizes."""
+    for i, (wb, w, s, z, k) in enumerate(all_configs()):
+        m = (1 << 17) if i % 2 == 0 else 100003
+        cfg = (wb, m, k, w, s, z)
+        f = O.Bloom(*cfg)
+        desc = D.bbf_desc(*cfg)
+        rng = np.random.default_rng(900 + i)
+        build = rng.integers(0, 2**31, 2000, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        ow = oracle_words(f, build)
+        probe = mixed_keys(build, 30000 + 2048 * (i % 2) + i, 0.3, 950 + i)
+        exp = f.batch_contains(ow, probe, 3)
+        keys = to_dev(probe)
+        for grid in (1, 3, 7, 0):
+            out, cnt = D.bbf_probe(desc, words, keys, 3, params=D.ProbeParams(0, 1024, grid, 0, 0))
+            torch.cuda.synchronize()
+            assert np.array_equal(D.positions(out, cnt), exp), (cfg, grid)
+
+
 def test_every_configuration_global_paths():
     """Every configuration through the global-memory kernels (filter larger
     than the LDS limit given): the lane-split kernel where the configuration
@@ -346,6 +370,109 @@ def test_cuckoo_partitioned_large_batch():
     _, _, lists = CB.cf_probe(o, buf, probe, min(16, CB.physical_cores()))
     exp = np.concatenate(lists).astype(np.uint64) + 3
     assert len(got) == len(exp) and np.array_equal(got, exp)
+
+
+# ---------------------------------------------------------------------------
+# cuckoo GPU build: concurrent inserts give a different table than the
+# sequential insert, so parity is "valid table + same probe semantics": the
+# oracle's placement check (each key's tag in one of its two buckets or the
+# victim cache, exactly one tag per key), every key found, and the GPU probe
+# list equal to the oracle's on the GPU-built bytes.
+# ---------------------------------------------------------------------------
+CF_GPU_BUILD = [(16, 4), (8, 4), (4, 4), (32, 4), (16, 2), (8, 2), (32, 2), (4, 2), (16, 1), (8, 1), (12, 1),
+                (4, 1), (32, 1)]
+
+
+def distinct_keys(n, seed):
+    rng = np.random.default_rng(seed)
+    keys = np.unique(rng.integers(0, 2**31, int(n * 1.2) + 10, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(keys)
+    return keys[:n]
+
+
+def gpu_cf_build(d, keys, seed, max_kicks=0, buf=None):
+    if buf is None:
+        buf = D.new_cf_table(d)
+    failed = D.cf_build(d, buf, to_dev(keys), seed=seed, max_kicks=max_kicks)
+    torch.cuda.synchronize()
+    return buf, int(failed.item())
+
+
+def check_gpu_built(d, o, buf, keys, seed, offset=0):
+    hb = buf.cpu().numpy()
+    assert o.placement_valid(hb, keys)
+    assert o.batch_contains(hb, keys).size == len(keys)       # no false negatives
+    probe = mixed_keys(keys, 30000, 0.4, seed + 1)
+    exp = o.batch_contains(hb, probe, offset)
+    for prm in (None, cf_part_params(9)):
+        out, cnt = D.cf_probe(d, buf, to_dev(probe), offset, params=prm)
+        torch.cuda.synchronize()
+        assert np.array_equal(D.positions(out, cnt), exp)
+    return hb
+
+
+@pytest.mark.parametrize("t,b", CF_GPU_BUILD)
+def test_cuckoo_gpu_build_valid_table(t, b):
+    """Power-of-two and MAGIC tables at loads the sequential insert fills
+    without the victim cache."""
+    load = {1: 0.3, 2: 0.4 if t == 4 else 0.8, 4: 0.8 if t == 4 else 0.9}[b]
+    bpb = (t * b + 7) // 8
+    for m, seed in (((1 << 12) * bpb * 8, t * 10 + b), (3001 * bpb * 8, t * 10 + b + 500)):
+        d = D.cf_desc(m, t, b)
+        o = O.Cuckoo(m, t, b)
+        keys = distinct_keys(int(load * d.bucket_cnt * b), seed)
+        buf, failed = gpu_cf_build(d, keys, seed)
+        assert failed == 0, (t, b, m)
+        check_gpu_built(d, o, buf, keys, seed, offset=7)
+
+
+def test_cuckoo_gpu_build_rejects_straddling_buckets():
+    d = D.cf_desc(1 << 16, 12, 4)
+    dev()
+    with pytest.raises(RuntimeError):
+        D.cf_build(d, D.new_cf_table(d), to_dev(np.arange(10, dtype=np.uint32)))
+
+
+def test_cuckoo_gpu_build_in_two_calls():
+    """A second call adds keys to the table the first call built."""
+    m = 5003 * 64
+    d = D.cf_desc(m, 16, 4)
+    o = O.Cuckoo(m, 16, 4)
+    keys = distinct_keys(int(0.93 * d.bucket_cnt * 4), 31)
+    half = len(keys) // 2
+    buf, f1 = gpu_cf_build(d, keys[:half], 31)
+    buf, f2 = gpu_cf_build(d, keys[half:], 32, buf=buf)
+    assert f1 == 0 and f2 == 0
+    check_gpu_built(d, o, buf, keys, 31)
+
+
+def test_cuckoo_gpu_build_overflow_accounts_for_every_key():
+    """Past capacity, every walk ends in a slot, the victim cache or the
+    failed count; a table whose failed count is 0 stays valid."""
+    d = D.cf_desc(64 * 64, 16, 4)           # 64 buckets, 256 slots
+    o = O.Cuckoo(64 * 64, 16, 4)
+    keys = distinct_keys(400, 5)
+    buf, failed = gpu_cf_build(d, keys, 5, max_kicks=50)
+    hb = buf.cpu().numpy()
+    assert o.victim(hb)[2]
+    assert failed > 0
+    assert o.count_occupied(hb) + failed == len(keys)
+    keys = distinct_keys(200, 6)
+    buf, failed = gpu_cf_build(d, keys, 6)
+    assert failed == 0
+    check_gpu_built(d, o, buf, keys, 6)
+
+
+def test_cuckoo_gpu_build_benchmark_shape():
+    """The benchmark table's shape at 1/256 scale: 16-bit tags, 4 slots,
+    2^18 buckets, 95% load (996,147 keys)."""
+    m = (1 << 18) * 64
+    d = D.cf_desc(m, 16, 4)
+    o = O.Cuckoo(m, 16, 4)
+    keys = distinct_keys(int(0.95 * d.bucket_cnt * 4), 12)
+    buf, failed = gpu_cf_build(d, keys, 12)
+    assert failed == 0
+    check_gpu_built(d, o, buf, keys, 12)
 
 
 # ---------------------------------------------------------------------------

@@ -473,3 +473,97 @@ def test_swar_hasvalue_vs_slot_compare():
                 x = v << (bits_ * j)
                 assert O.swar_hasvalue(bits_, x, v)
                 assert not O.swar_hasvalue(bits_, x << (bits_ * 4), v) or bits_ * 8 > 64
+
+
+def _cf_sequential(m, t, b, n, seed):
+    cf = O.Cuckoo(m, t, b)
+    buf = cf.new_buffer()
+    rng = np.random.default_rng(seed)
+    keys = np.unique(rng.integers(0, 2**31, 2 * n, dtype=np.uint64).astype(np.uint32))[:n]
+    rng.shuffle(keys)
+    assert cf.insert(buf, keys, O.MT19937(seed))
+    return cf, buf, keys
+
+
+@pytest.mark.parametrize("m,t,b,load", [(1 << 17, 16, 4, 0.93), (3001 * 16, 8, 2, 0.8), (1 << 14, 4, 4, 0.5),
+                                        (5003 * 32, 32, 1, 0.35), (4099 * 48, 12, 4, 0.9)])
+def test_cuckoo_placement_check_accepts_sequential_builds(m, t, b, load):
+    """Any table the reference's sequential insert builds passes the
+    placement check, victim cache included."""
+    cf = O.Cuckoo(m, t, b)
+    n = int(load * cf.bucket_cnt * b)
+    cf, buf, keys = _cf_sequential(m, t, b, n, seed=t + b)
+    assert cf.placement_valid(buf, keys)
+    assert not cf.placement_valid(buf, keys[:-1])          # one tag too many
+    if len(keys) > 1:
+        other = keys.copy()
+        other[0] = np.uint32(0x80000000 | int(other[0]))   # a key that was never inserted
+        assert cf.placement_valid(buf, other) == (cf.tag(int(other[0])) == cf.tag(int(keys[0])) and
+                                                  {cf.i1(int(other[0])), cf.alt(cf.i1(int(other[0])), cf.tag(int(other[0])))}
+                                                  == {cf.i1(int(keys[0])), cf.alt(cf.i1(int(keys[0])), cf.tag(int(keys[0])))})
+
+
+def test_cuckoo_placement_check_rejects_planted_mistakes():
+    """Mistakes planted in an oracle-built 16-bit / 4-slot table (slot j of
+    bucket i = u16 at byte 8i + 2j): a tag moved to a bucket of neither of its
+    key's candidates, a dropped tag, a duplicated tag replacing another key's,
+    and a victim entry pointing at the wrong bucket all fail the check."""
+    m = 1 << 16                                # 2^10 buckets
+    cf = O.Cuckoo(m, 16, 4)
+    n = int(0.9 * cf.bucket_cnt * 4)
+    cf, buf, keys = _cf_sequential(m, 16, 4, n, seed=77)
+    assert cf.placement_valid(buf, keys)
+    slots = lambda b_: b_[: cf.bucket_cnt * 8].view(np.uint16).reshape(cf.bucket_cnt, 4)  # noqa: E731
+    s0 = slots(buf)
+    k0 = next(int(k) for k in keys.tolist() if (s0[cf.i1(k)] == 0).any())   # first bucket has room
+    t0, a0 = cf.tag(k0), cf.i1(k0)
+    b0 = cf.alt(a0, t0)
+    # 1. move key 0's tag copy to an empty slot of a foreign bucket
+    bad = buf.copy()
+    s = slots(bad)
+    where = [(i, j) for i in (a0, b0) for j in range(4) if s[i, j] == t0]
+    assert where
+    empty = [(i, j) for i in range(cf.bucket_cnt) for j in range(4)
+             if s[i, j] == 0 and i not in (a0, b0)]
+    i, j = where[0]
+    s[i, j] = 0
+    s[empty[0]] = t0
+    # the move is invalid unless another key with the same tag owns that bucket
+    owners = [k for k in keys.tolist() if cf.tag(k) == t0 and empty[0][0] in (cf.i1(k), cf.alt(cf.i1(k), t0))]
+    assert not owners
+    assert not cf.placement_valid(bad, keys)
+    # 2. drop a tag
+    bad = buf.copy()
+    slots(bad)[i, j] = 0
+    assert not cf.placement_valid(bad, keys)
+    # 3. duplicate key 0's tag over key 1's (same count, one key unexplained)
+    bad = buf.copy()
+    s = slots(bad)
+    k1 = next(int(k) for k in keys.tolist() if cf.tag(int(k)) != t0 and a0 not in
+              (cf.i1(int(k)), cf.alt(cf.i1(int(k)), cf.tag(int(k)))))
+    t1, c1 = cf.tag(k1), cf.i1(k1)
+    w1 = [(ii, jj) for ii in (c1, cf.alt(c1, t1)) for jj in range(4) if s[ii, jj] == t1]
+    assert w1 and t1 != t0
+    s[w1[0]] = 0
+    free = [(a0, jj) for jj in range(4) if s[a0, jj] == 0]
+    s[free[0]] = t0
+    assert cf.count_occupied(bad) == cf.count_occupied(buf)
+    assert not cf.placement_valid(bad, keys)
+    # 4. victim cache at the wrong index
+    cfo = O.Cuckoo(64 * 64, 16, 4)             # 64 buckets: overfill into the victim cache
+    vb = cfo.new_buffer()
+    g = O.MT19937(3)
+    ins = []
+    for k in (np.arange(1, 400, dtype=np.uint32) * np.uint32(2654435761 & 0x7FFFFFFF)).tolist():
+        if not cfo.insert(vb, [k], g):
+            break
+        ins.append(k)
+    assert cfo.victim(vb)[2]
+    assert cfo.placement_valid(vb, np.array(ins, np.uint32))
+    vi, vt, _ = cfo.victim(vb)
+    owners = [k for k in ins if cfo.tag(k) == vt]
+    cand = {x for k in owners for x in (cfo.i1(k), cfo.alt(cfo.i1(k), vt))}
+    wrong = next(x for x in range(64) if x not in cand)
+    vb2 = vb.copy()
+    vb2[cfo.d.victim_offset: cfo.d.victim_offset + 4] = np.frombuffer(np.uint32(wrong).tobytes(), np.uint8)
+    assert not cfo.placement_valid(vb2, np.array(ins, np.uint32))
