@@ -154,7 +154,7 @@ GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal"
                 2: "4 lanes / 64 B sector, 1x16 B each", 3: "as 2, non-temporal",
                 4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector",
                 6: "as 2, uniform random sectors", 7: "as 0, uniform random sectors",
-                8: "LDS-staged buffer, 1 lane / uniform random 4 B word"}
+                8: "LDS-staged buffer, 1 lane / pseudo-random 4 B word (per-lane LCG)"}
 
 
 def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
@@ -198,6 +198,25 @@ def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, wa
             "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
             "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
             "call_ms": st, "matches": matches}
+
+
+def run_regimes(torch, D, steps_arg):
+    """The filter-size sweep: 16 KiB (LDS), 4 MiB (L2), 128 MiB (Infinity
+    Cache) and the plain register-blocked 128 KiB filter, with the LDS
+    random-read rate beside the two LDS-staged ones."""
+    log("bench: filter-size regimes")
+    steps = max(3, steps_arg // 2)
+    regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 16 KiB"),
+            regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 4 MiB"),
+            regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 128 MiB"),
+            regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, 1,
+                   "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
+    for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
+        lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
+        r["lds_random_4B_reads"] = {"G_reads_per_s": lds["G_sectors_per_s"], "buffer_bytes": 1 << lb,
+                                    "probe_vs_lds_reads": round(r["kernel_G_probes_per_s"]
+                                                                / lds["G_sectors_per_s"], 3)}
+    return regs
 
 
 FIXED_SEMI_JOIN_CFG = (32, 6, 16, 16, 2)   # (word_bits, k, w, s, z): u32 words, w=16, z=2, k=6
@@ -534,6 +553,8 @@ def main():
     ap.add_argument("--gather-only", action="store_true",
                     help="only run the random-sector gather microbenchmark (all access patterns)")
     ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
+    ap.add_argument("--regimes-only", action="store_true",
+                    help="only run the filter-size sweep (16 KiB / 4 MiB / 128 MiB and the plain 128 KiB filter)")
     ap.add_argument("--batched", type=int, default=0, metavar="FILTERS",
                     help="only run the batched semi-join workload with this many filters (4096 = full)")
     ap.add_argument("--cuckoo", action="store_true",
@@ -572,6 +593,10 @@ def main():
     if args.batched:
         print(json.dumps({"batched_semi_join": run_batched(torch, D, args.batched, max(2, args.steps // 4))}),
               flush=True)
+        return
+
+    if args.regimes_only:
+        print(json.dumps({"regimes": run_regimes(torch, D, args.steps)}), flush=True)
         return
 
     if args.gather_only:
@@ -669,19 +694,7 @@ def main():
         "probe_phase_vs_gather": round(n / (kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
         "note": "gfx950 fills a whole 128 B L2 line per random 64 B block (TCC_EA0_RDREQ_128B = all reads)"}
     if not args.no_regimes:
-        log("bench: filter-size regimes")
-        steps = max(3, args.steps // 2)
-        regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 16 KiB"),
-                regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 4 MiB"),
-                regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 128 MiB"),
-                regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, 1,
-                       "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
-        for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
-            lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
-            r["lds_random_4B_reads"] = {"G_reads_per_s": lds["G_sectors_per_s"], "buffer_bytes": 1 << lb,
-                                        "probe_vs_lds_reads": round(r["kernel_G_probes_per_s"]
-                                                                    / lds["G_sectors_per_s"], 3)}
-        result["regimes"] = regs
+        result["regimes"] = run_regimes(torch, D, args.steps)
     print(json.dumps(result), flush=True)
 
 

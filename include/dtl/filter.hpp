@@ -14,7 +14,8 @@
 //   * filter words, keys and match-position buffers of the blocked Bloom
 //     filters are DEVICE pointers (HBM); every operation runs on the GPU;
 //   * the cuckoo filter is built on the host (insert / batch_insert take host
-//     table memory, as the reference builds it) and probed on the GPU
+//     table memory, as the reference builds it) or on the GPU
+//     (batch_insert_device, concurrent inserts) and probed on the GPU
 //     (contains / batch_contains take the device copy of the table);
 //   * the table buffer holds the victim cache after the buckets: allocate
 //     buffer_size_in_bytes(), not size_in_bytes();
@@ -305,6 +306,19 @@ struct cuckoofilter {
     detail::throw_status(cf_build_host(&desc, h_table, h_keys, key_cnt, static_cast<u32>(rng()), &inserted),
                          "cf_build_host");
     return inserted == key_cnt;
+  }
+
+  // GPU build (d_table: buffer_size_in_bytes() zeroed device bytes; d_keys
+  // device keys): every key inserted at once (cf_build).  True if every walk
+  // found a place, i.e. the table holds every key (the reference's
+  // batch_insert returns true while its victim cache takes the overflow).
+  // Not available for 12-bit tags with 2 or 4 slots (std::invalid_argument).
+  u1 batch_insert_device(word_t* d_table, const key_t* d_keys, u64 key_cnt, hipStream_t stream = nullptr) {
+    u64* failed = reinterpret_cast<u64*>(static_cast<char*>(detail::scratch64()) + 32);
+    detail::check_hip(hipMemsetAsync(failed, 0, sizeof(u64), stream), "hipMemsetAsync");
+    detail::throw_status(cf_build(&desc, d_table, d_keys, key_cnt, static_cast<u32>(rng()), 0, failed, stream),
+                         "cf_build");
+    return detail::read_count(failed, stream) == 0;
   }
 
   // GPU probe (d_table: device copy of the table buffer).

@@ -130,6 +130,24 @@ static void cuckoo_test() {
   std::vector<uint32_t> table(raw.buffer_size_in_bytes() / 4 + 1, 0);
   CHECK(raw.batch_insert(table.data(), keys.data(), static_cast<uint32_t>(keys.size())));
   CHECK(raw.count_occupied_slots(table.data()) == keys.size());
+  // GPU build of the same keys into a device table: every key found by the GPU probe
+  void* d_table = nullptr;
+  uint32_t* d_keys = nullptr;
+  uint32_t* d_pos = nullptr;
+  CHECK(hipMalloc(&d_table, raw.buffer_size_in_bytes() + 16) == hipSuccess);
+  CHECK(hipMalloc(reinterpret_cast<void**>(&d_keys), keys.size() * 4) == hipSuccess);
+  CHECK(hipMalloc(reinterpret_cast<void**>(&d_pos), keys.size() * 4) == hipSuccess);
+  CHECK(hipMemset(d_table, 0, raw.buffer_size_in_bytes() + 16) == hipSuccess);
+  CHECK(hipMemcpy(d_keys, keys.data(), keys.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+  CHECK(raw.batch_insert_device(static_cast<uint32_t*>(d_table), d_keys, keys.size()));
+  CHECK(raw.batch_contains(static_cast<const uint32_t*>(d_table), d_keys, static_cast<uint32_t>(keys.size()),
+                           d_pos, 0) == keys.size());
+  std::vector<uint32_t> gtable(table.size(), 0);
+  CHECK(hipMemcpy(gtable.data(), d_table, raw.buffer_size_in_bytes(), hipMemcpyDeviceToHost) == hipSuccess);
+  CHECK(raw.count_occupied_slots(gtable.data()) == keys.size());
+  (void)hipFree(d_table);
+  (void)hipFree(d_keys);
+  (void)hipFree(d_pos);
   bool thrown = false;
   try {
     dtl::cuckoofilter::cuckoofilter bad(1u << 20, 7, 4);
