@@ -432,6 +432,38 @@ __global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_s
 }
 
 // ---------------------------------------------------------------------------
+// Lane-split probe of a filter staged in LDS (persistent grid, 256 threads):
+// the lanes of a key read the 16-byte pieces of its block from LDS, one each,
+// so a wave's LDS read covers KPI whole blocks in contiguous 16-byte pieces
+// instead of 64 scattered 16-byte reads.
+// ---------------------------------------------------------------------------
+template <class C>
+__global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_lds_split(
+    const typename C::word_t* __restrict__ filter, uint32_t filter_bytes, const uint32_t* __restrict__ keys,
+    uint32_t n, AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt, uint32_t ntiles) {
+  using W = typename C::word_t;
+  constexpr int BT = kGlobalThreads;
+  extern __shared__ uint4 lds_filter[];
+  __shared__ uint32_t red[BT / 64];
+  const uint32_t tid = threadIdx.x;
+  {
+    const uint32_t n16 = filter_bytes / 16;  // blocks of this kernel are whole 16-byte pieces
+    const uint4* src = reinterpret_cast<const uint4*>(filter);
+    for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
+  }
+  __syncthreads();
+  const W* lds = reinterpret_cast<const W*>(lds_filter);
+  const SplitLane<C> sl(tid & (C::split.lanes - 1));
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t cnt = split_probe_tile<C, BT>(sl, lds, keys, n, addr, static_cast<uint64_t>(t) * kTileKeys,
+                                                 bits + static_cast<uint64_t>(t) * kTileWords);
+    const uint32_t total = block_sum_waves<BT>(cnt, red);
+    if (tid == 0) tile_cnt[t] = total;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
 // Workgroup b probes tiles b, b + grid, b + 2 grid, ...; it requests the keys
 // of G of its tiles at once (G x ROUNDS keys per lane, 8 with the 1024-thread
