@@ -35,7 +35,6 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
   __shared__ uint32_t red[BT / 64];
   __shared__ uint32_t s_start;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const W* lds = reinterpret_cast<const W*>(lds_filter);
   uint32_t staged = 0xFFFFFFFFu;
   // per-lane multipliers of the lane-split path (nothing for other configurations)
   [[maybe_unused]] const std::conditional_t<C::split.ok, SplitLane<C>, NoSplitLane> sl(lane & (C::split.lanes - 1));
@@ -60,12 +59,7 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
       const W* filter = static_cast<const W*>(T.words);
       const bool use_lds = fbytes <= lds_max;
       if (use_lds && ti != staged) {
-        const uint32_t n16 = fbytes / 16;
-        const uint4* src = reinterpret_cast<const uint4*>(filter);
-        for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
-        uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds_filter);
-        const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
-        for (uint32_t i = n16 * 4 + tid; i < fbytes / 4; i += BT) dst32[i] = src32[i];
+        LdsLayout<C>::stage(lds_filter, filter, fbytes, tid, BT);
         staged = ti;
         __syncthreads();
       }
@@ -97,7 +91,7 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const uint32_t b = block_of(key[u] * kMul.v[0], addr);
-            hit[u] = test_ptr<C>(key[u], lds + static_cast<uint32_t>(b * C::w)) & valid[u];
+            hit[u] = LdsLayout<C>::test(key[u], b, lds_filter) & valid[u];
           }
         } else {
           W blk[U][C::w];

@@ -432,36 +432,59 @@ __global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_s
 }
 
 // ---------------------------------------------------------------------------
-// Lane-split probe of a filter staged in LDS (persistent grid, 256 threads):
-// the lanes of a key read the 16-byte pieces of its block from LDS, one each,
-// so a wave's LDS read covers KPI whole blocks in contiguous 16-byte pieces
-// instead of 64 scattered 16-byte reads.
+// LDS layout of a staged filter.  Blocks of 64 or 128 bytes start on 64- or
+// 128-byte boundaries, so the j-th 16-byte piece of every block falls on one
+// of only 4 (2) groups of banks of the 256-byte bank row, and 64 lanes
+// reading piece j of random blocks conflict heavily (measured 4.8 conflict
+// cycles per active LDS cycle for u64 w8 s8 k8).  Where the test reads
+// whole, statically indexed words, the staged copy rotates the pieces of
+// block b by (b >> R) mod P (P pieces per block, R = log2(256 / block
+// bytes)), which spreads piece j of different blocks over all bank groups;
+// the probe reads the pieces back in logical order, so the test is unchanged.
 // ---------------------------------------------------------------------------
 template <class C>
-__global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_lds_split(
-    const typename C::word_t* __restrict__ filter, uint32_t filter_bytes, const uint32_t* __restrict__ keys,
-    uint32_t n, AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt, uint32_t ntiles) {
-  using W = typename C::word_t;
-  constexpr int BT = kGlobalThreads;
-  extern __shared__ uint4 lds_filter[];
-  __shared__ uint32_t red[BT / 64];
-  const uint32_t tid = threadIdx.x;
-  {
-    const uint32_t n16 = filter_bytes / 16;  // blocks of this kernel are whole 16-byte pieces
-    const uint4* src = reinterpret_cast<const uint4*>(filter);
-    for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
-  }
-  __syncthreads();
-  const W* lds = reinterpret_cast<const W*>(lds_filter);
-  const SplitLane<C> sl(tid & (C::split.lanes - 1));
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t cnt = split_probe_tile<C, BT>(sl, lds, keys, n, addr, static_cast<uint64_t>(t) * kTileKeys,
-                                                 bits + static_cast<uint64_t>(t) * kTileWords);
-    const uint32_t total = block_sum_waves<BT>(cnt, red);
-    if (tid == 0) tile_cnt[t] = total;
-    __syncthreads();
-  }
+constexpr bool lds_swizzled() {
+  constexpr uint32_t bytes = C::w * sizeof(typename C::word_t);
+  if (bytes != 64 && bytes != 128) return false;
+  for (int g = 0; g < C::sched.ngroups; ++g)
+    if (C::sched.gdyn[g].nbits != 0) return false;
+  return true;
 }
+
+template <class C>
+struct LdsLayout {
+  static constexpr bool kSwz = lds_swizzled<C>();
+  static constexpr uint32_t P = C::w * sizeof(typename C::word_t) / 16;  // 16-byte pieces per block
+  static constexpr uint32_t R = P == 8 ? 1u : 2u;
+  // LDS piece index of piece p of block b (swizzled layouts only)
+  __device__ __forceinline__ static uint32_t slot(uint32_t b, uint32_t p) {
+    return b * P + ((p + (b >> R)) & (P - 1));
+  }
+  __device__ __forceinline__ static void stage(uint4* __restrict__ lds, const void* filter, uint32_t filter_bytes,
+                                               uint32_t tid, uint32_t nthreads) {
+    const uint32_t n16 = filter_bytes / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(filter);
+    if constexpr (kSwz) {
+      for (uint32_t i = tid; i < n16; i += nthreads) lds[slot(i / P, i % P)] = src[i];
+    } else {
+      for (uint32_t i = tid; i < n16; i += nthreads) lds[i] = src[i];
+      uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds);
+      const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
+      for (uint32_t i = n16 * 4 + tid; i < filter_bytes / 4; i += nthreads) dst32[i] = src32[i];
+    }
+  }
+  __device__ __forceinline__ static bool test(uint32_t key, uint32_t b, const uint4* __restrict__ lds) {
+    using W = typename C::word_t;
+    if constexpr (kSwz) {
+      uint4 piece[P];
+#pragma unroll
+      for (uint32_t p = 0; p < P; ++p) piece[p] = lds[slot(b, p)];
+      return test_ptr<C>(key, reinterpret_cast<const W*>(piece));
+    } else {
+      return test_ptr<C>(key, reinterpret_cast<const W*>(lds) + static_cast<uint32_t>(b * C::w));
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
@@ -481,23 +504,14 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
                                                       AddrArgs addr, uint64_t* __restrict__ bits,
                                                       uint32_t* __restrict__ tile_cnt,
                                                       uint32_t ntiles) {
-  using W = typename C::word_t;
   constexpr int ROUNDS = kTileKeys / BT;
   constexpr int G = ROUNDS >= 8 ? 1 : 8 / ROUNDS;  // tiles whose keys are requested together
   constexpr int NW = BT / 64;
   extern __shared__ uint4 lds_filter[];
   __shared__ uint32_t red[G][NW];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  {
-    const uint32_t n16 = filter_bytes / 16;
-    const uint4* src = reinterpret_cast<const uint4*>(filter);
-    for (uint32_t i = tid; i < n16; i += BT) lds_filter[i] = src[i];
-    uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds_filter);
-    const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
-    for (uint32_t i = n16 * 4 + tid; i < filter_bytes / 4; i += BT) dst32[i] = src32[i];
-  }
+  LdsLayout<C>::stage(lds_filter, filter, filter_bytes, tid, BT);
   __syncthreads();
-  const W* lds = reinterpret_cast<const W*>(lds_filter);
   for (uint32_t t0 = blockIdx.x; t0 < ntiles; t0 += G * gridDim.x) {
     uint32_t key[G][ROUNDS];
 #pragma unroll
@@ -518,7 +532,7 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
       for (int r = 0; r < ROUNDS; ++r) {
         const uint64_t i = tile_base + static_cast<uint64_t>(r) * BT + tid;
         const uint32_t b = block_of(key[g][r] * kMul.v[0], addr);
-        const bool hit = test_ptr<C>(key[g][r], lds + static_cast<uint32_t>(b * C::w)) & (i < n);
+        const bool hit = LdsLayout<C>::test(key[g][r], b, lds_filter) & (i < n);
         const uint64_t m = __ballot(hit);
         if (lane == 0 && tv) store_bits(bits, (tile_base + static_cast<uint64_t>(r) * BT) / 64 + wave, m);
         cnt += __popcll(m);

@@ -45,39 +45,9 @@ hipError_t launch_probe_lds(const BbfProbeLaunch& L) {
 }
 
 template <class C>
-hipError_t launch_probe_lds_split(const BbfProbeLaunch& L) {
-  using W = typename C::word_t;
-  auto fn = k_bbf_probe_lds_split<C>;
-  const uint32_t lds_bytes = (L.filter_bytes + 15u) & ~15u;
-  hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
-  if (ae != hipSuccess) return ae;
-  uint32_t grid = L.lds_grid;
-  if (grid == 0) {
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kGlobalThreads, lds_bytes);
-    if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    grid = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(std::max(L.cu_count, 1));
-  }
-  grid = std::max(1u, std::min(grid, L.ntiles));
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(kGlobalThreads), lds_bytes, L.stream, static_cast<const W*>(L.words),
-                     L.filter_bytes, L.keys, L.n, make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post),
-                     L.bits, L.tile_cnt, L.ntiles);
-  return hipGetLastError();
-}
-
-template <class C>
 hipError_t launch_probe(const BbfProbeLaunch& L) {
   using W = typename C::word_t;
   if (L.use_lds) {
-    // Lane split from LDS only for 128-byte blocks (8 lanes per key) whose
-    // test reads every word of the block, in filters of up to 64 KiB: there
-    // it measured 1.40x (16 KiB) and 1.23x (64 KiB) the one-lane kernel
-    // (u64 w16 s16 k16), but 0.89x at 128 KiB (one 256-thread workgroup per
-    // CU), 0.77x where the one-lane test reads half the words (u64 w16 s8
-    // k8) and 0.94x with 64-byte blocks (u64 w8 s8 k8).
-    if constexpr (C::split.ok && C::split.lanes == 8 && C::sched.ngroups == C::w) {
-      if (L.split && L.filter_bytes <= 64u * 1024u) return launch_probe_lds_split<C>(L);
-    }
     // 1024-thread workgroups only for 32-bit words: with 64-bit words that
     // shape lost whole 64-key ballot words now and then on gfx950 (ROCm 7.2;
     // cause not found: it persists with the blocks read from global memory

@@ -139,8 +139,8 @@ def test_every_configuration_lds_tile_groups():
     of 4 tiles at a time; 64-bit words run 256-thread workgroups): grids of
     1, 3 and 7 workgroups and the default over 15 or 16 tiles, so that tile
     groups end past the last tile at every position; POW2 and MAGIC sizes;
-    with the default kernel choice (the lane-split LDS kernel for 128-byte
-    blocks whose test reads every word) and with the one-lane kernel."""
+    default flags and DTL_PROBE_ONE_LANE (both take the LDS kernel; 64- and
+    128-byte blocks with statically indexed words use its swizzled layout)."""
     for i, (wb, w, s, z, k) in enumerate(all_configs()):
         m = (1 << 17) if i % 2 == 0 else 100003
         cfg = (wb, m, k, w, s, z)
@@ -154,7 +154,7 @@ def test_every_configuration_lds_tile_groups():
         exp = f.batch_contains(ow, probe, 3)
         keys = to_dev(probe)
         for grid in (1, 3, 7, 0):
-            for flags in (0, D.PROBE_ONE_LANE):  # lane-split LDS kernel where chosen, and the one-lane one
+            for flags in (0, D.PROBE_ONE_LANE):
                 out, cnt = D.bbf_probe(desc, words, keys, 3, params=D.ProbeParams(0, 1024, grid, 0, flags))
                 torch.cuda.synchronize()
                 assert np.array_equal(D.positions(out, cnt), exp), (cfg, grid, flags)
