@@ -489,9 +489,10 @@ struct LdsLayout {
 // ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
 // Workgroup b probes tiles b, b + grid, b + 2 grid, ...; it requests the keys
-// of G of its tiles at once (G x ROUNDS keys per lane, 8 with the 1024-thread
-// block that large filters leave one of per CU), then tests them, so that
-// key loads overlap even at one workgroup per CU.  A tile past the end is
+// of G of its tiles at once (G x ROUNDS = 16 keys per lane: 8 tiles with the
+// 1024-thread block that large filters leave one of per CU, 2 with 256
+// threads), then tests them, so that key loads overlap even at one
+// workgroup per CU.  A tile past the end is
 // probed like the others (its keys are clamped, its hits masked) and only its
 // stores are skipped: the same loop with a uniform early exit per tile lost
 // hits of the group's first tile now and then on gfx950 (ROCm 7.2), which
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
                                                       uint32_t* __restrict__ tile_cnt,
                                                       uint32_t ntiles) {
   constexpr int ROUNDS = kTileKeys / BT;
-  constexpr int G = ROUNDS >= 8 ? 1 : 8 / ROUNDS;  // tiles whose keys are requested together
+  constexpr int G = ROUNDS >= 16 ? 1 : 16 / ROUNDS;  // tiles whose keys are requested together
   constexpr int NW = BT / 64;
   extern __shared__ uint4 lds_filter[];
   __shared__ uint32_t red[G][NW];

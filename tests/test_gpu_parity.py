@@ -136,7 +136,7 @@ def test_every_configuration_parity():
 def test_every_configuration_lds_tile_groups():
     """Every configuration through the LDS kernel asked for 1024-thread
     workgroups (taken for 32-bit words, whose workgroups then take the keys
-    of 4 tiles at a time; 64-bit words run 256-thread workgroups): grids of
+    of 8 tiles at a time; 64-bit words run 256-thread workgroups, which take 2): grids of
     1, 3 and 7 workgroups and the default over 15 or 16 tiles, so that tile
     groups end past the last tile at every position; POW2 and MAGIC sizes;
     default flags and DTL_PROBE_ONE_LANE (both take the LDS kernel; 64- and
