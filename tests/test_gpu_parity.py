@@ -466,6 +466,23 @@ def test_cuckoo_gpu_build_overflow_accounts_for_every_key():
     check_gpu_built(d, o, buf, keys, 6)
 
 
+@pytest.mark.parametrize("buckets,t,b", [(300007, 16, 4), (250007, 32, 4), (200003, 8, 2)])
+def test_cuckoo_gpu_build_magic_high_load(buckets, t, b):
+    """MAGIC tables at high load (95% for 4 slots, 85% for 2): the 16-byte
+    buckets of 32-bit tags (two compare-and-swap words per bucket) and
+    sub-word buckets sharing a 4-byte word with their neighbours.  Odd
+    bucket counts: with an even one the reference's MAGIC alternate index
+    caps the fill at 74-85% (its sequential insert overflows too)."""
+    bpb = (t * b + 7) // 8
+    d = D.cf_desc(buckets * bpb * 8, t, b)
+    assert d.addr_mode == D.ADDR_MAGIC and d.bucket_cnt % 2 == 1
+    o = O.Cuckoo(buckets * bpb * 8, t, b)
+    keys = distinct_keys(int((0.95 if b == 4 else 0.85) * d.bucket_cnt * b), buckets)
+    buf, failed = gpu_cf_build(d, keys, 3)
+    assert failed == 0
+    check_gpu_built(d, o, buf, keys, 4)
+
+
 def test_cuckoo_gpu_build_benchmark_shape():
     """The benchmark table's shape at 1/256 scale: 16-bit tags, 4 slots,
     2^18 buckets, 95% load (996,147 keys)."""

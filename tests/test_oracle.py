@@ -567,3 +567,37 @@ def test_cuckoo_placement_check_rejects_planted_mistakes():
     vb2 = vb.copy()
     vb2[cfo.d.victim_offset: cfo.d.victim_offset + 4] = np.frombuffer(np.uint32(wrong).tobytes(), np.uint8)
     assert not cfo.placement_valid(vb2, np.array(ins, np.uint32))
+
+
+def test_cuckoo_magic_even_bucket_count_alternates_are_even():
+    """With an even MAGIC bucket count c, i1 = h0 mod c, the tag and
+    s = (tag * 0x5bd1e995) mod c all carry the key's parity
+    (cuckoofilter_logic.hpp:179-186, 241-250), so alt = (-s - i1) mod c is
+    always even and even keys see only even buckets; the reference's insert
+    then overflows at 74-85% load.  Odd bucket counts are unaffected."""
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 2**31, 4000, dtype=np.uint64).astype(np.uint32).tolist()
+    even = O.Cuckoo(60000 * 64, 16, 4)
+    assert even.bucket_cnt % 2 == 0 and even.d.addr.mode == O.MAGIC
+    for k in keys:
+        i = even.i1(k)
+        assert i % 2 == k % 2
+        assert even.alt(i, even.tag(k)) % 2 == 0
+    odd = O.Cuckoo(70001 * 64, 16, 4)
+    assert odd.bucket_cnt % 2 == 1
+    share = np.mean([odd.alt(odd.i1(k), odd.tag(k)) % 2 == 0 for k in keys])
+    assert 0.45 < share < 0.55
+    # the capacity consequence, with the reference's sequential insert
+    rng2 = np.random.default_rng(3)
+    ks = np.unique(rng2.integers(0, 2**31, 300000, dtype=np.uint64).astype(np.uint32))
+    rng2.shuffle(ks)
+    for cf, full in ((O.Cuckoo(20000 * 64, 16, 4), False), (O.Cuckoo(20001 * 64, 16, 4), True)):
+        n = int(0.95 * cf.bucket_cnt * 4)
+        buf = cf.new_buffer()
+        g = O.MT19937(1)
+        inserted = 0
+        for k in ks[:n].tolist():
+            if not cf.insert(buf, [k], g):
+                break
+            inserted += 1
+        assert (inserted == n) == full, (cf.bucket_cnt, inserted, n)
