@@ -716,3 +716,15 @@ def test_batched_mixed_configurations():
         for g, e in zip(plan.results(), expect):
             assert np.array_equal(g, e)
         plan.close()
+
+
+# ---------------------------------------------------------------------------
+# roofline microbenchmark: every access pattern runs (it reports only time)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mode,log2_bytes", [(m, 22) for m in range(8)] + [(8, 14), (8, 17)])
+def test_gather_bench_modes_run(mode, log2_bytes):
+    dev()
+    buf = torch.randint(0, 2**31, (1 << (log2_bytes - 2),), dtype=torch.int32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.int64, device="cuda")
+    D.gather_bench(buf, mode, (1 << 20) + 5, sink)
+    torch.cuda.synchronize()
