@@ -224,6 +224,23 @@ static bool desc_ok(const dtl_bbf_desc* d) {
 
 static uint32_t block_bytes(const dtl_bbf_desc* d) { return d->w * d->word_bits / 8; }
 
+// Default choice of the partitioned probe (Bloom and cuckoo filters): the
+// batch must be large enough for the partition passes to pay for
+// themselves, and the smaller the filter, the larger the batch it takes.
+// Measured on MI355X (u64 w8 s8 k8 Bloom filter; 16-bit / 4-slot cuckoo
+// table at 95% load; 1% hits), partitioned vs direct G probes/s:
+//   Bloom  8 MiB: 2^26 108 / 110, 2^28 123 / 111;  16 MiB: 2^24 73 / 73,
+//          2^26 107 / 74;  32 MiB: 2^24 72 / 61;  64 MiB: 2^24 72 / 55;
+//   cuckoo 8 MiB: 2^28 59 / 58;  16 MiB: 2^24 38 / 36, 2^26 52 / 37;
+//          32 MiB: 2^24 37 / 31;  64 MiB: 2^24 37 / 28.
+static bool partition_pays(uint64_t filter_bytes, uint64_t n, bool cuckoo) {
+  if (filter_bytes >= (64ull << 20)) return n >= (1ull << 22);
+  if (filter_bytes >= (32ull << 20)) return n >= (1ull << 23);
+  if (filter_bytes >= (16ull << 20)) return n >= (1ull << 25);
+  if (filter_bytes >= (8ull << 20)) return !cuckoo && n >= (1ull << 27);
+  return false;
+}
+
 static bool aligned_to(const void* p, uint32_t a) {
   return (reinterpret_cast<uintptr_t>(p) % a) == 0;
 }
@@ -395,7 +412,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   uint32_t slice_shift = 0, nparts = 0;
   if (!use_lds && !(flags & DTL_PROBE_DIRECT) && e->probe_part) {
     const bool want = (flags & DTL_PROBE_PARTITIONED) ||
-                      (d->size_bytes >= (64ull << 20) && n >= (1ull << 22));
+                      partition_pays(d->size_bytes, n, false);
     // slices of 512 KiB, 1 MiB from 128 MiB on, 2 MiB from 512 MiB on
     // (measured on MI355X with 2^26..2^28 probe keys)
     const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
@@ -573,7 +590,7 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
   // partitioned probe: two passes over keys grouped by bucket slice
   const uint64_t table_bytes = d->victim_offset;
   const bool want = !(flags & DTL_PROBE_DIRECT) &&
-                    ((flags & DTL_PROBE_PARTITIONED) || (table_bytes >= (64ull << 20) && n >= (1ull << 22)));
+                    ((flags & DTL_PROBE_PARTITIONED) || partition_pays(table_bytes, n, true));
   uint32_t slice_shift = 0, nparts = 0;
   if (want) {
     const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
