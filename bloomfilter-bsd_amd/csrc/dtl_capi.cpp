@@ -413,11 +413,12 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   if (!use_lds && !(flags & DTL_PROBE_DIRECT) && e->probe_part) {
     const bool want = (flags & DTL_PROBE_PARTITIONED) ||
                       partition_pays(d->size_bytes, n, false);
-    // slices of 512 KiB, 1 MiB from 128 MiB on, 2 MiB from 512 MiB on
-    // (measured on MI355X with 2^26..2^28 probe keys)
+    // slices of 512 KiB, 2 MiB from 64 MiB on (measured on MI355X, u64 w8 s8
+    // k8, 2^28 keys, G probes/s for 512 KiB / 1 MiB / 2 MiB slices: 64 MiB
+    // 115 / 120 / 120, 128 MiB 109 / 116 / 118, 256 MiB 104 / 110 / 114;
+    // below 64 MiB the three are within 2%)
     const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
-                                : d->size_bytes >= (512ull << 20) ? 21u
-                                : d->size_bytes >= (128ull << 20) ? 20u : 19u;
+                                : d->size_bytes >= (64ull << 20) ? 21u : 19u;
     const uint32_t bb_log2 = floor_log2(block_bytes(d));
     slice_shift = slice_log2 > bb_log2 ? slice_log2 - bb_log2 : 0u;
     while (slice_shift < 31 && ((static_cast<uint64_t>(d->block_cnt) + (1ull << slice_shift) - 1) >> slice_shift) > kPartMax)
@@ -593,9 +594,11 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
                     ((flags & DTL_PROBE_PARTITIONED) || partition_pays(table_bytes, n, true));
   uint32_t slice_shift = 0, nparts = 0;
   if (want) {
+    // 2 MiB slices from 64 MiB on, as for Bloom filters (16-bit / 4-slot
+    // table, 2^28 keys, 512 KiB / 1 MiB / 2 MiB: 64 MiB 56.6 / 57.3 / 57.3,
+    // 128 MiB 53.6 / 55.8 / 56.4, 512 MiB 44.5 / 48.5 / 51.3 G probes/s)
     const uint32_t slice_log2 = (params && params->slice_log2) ? params->slice_log2
-                                : table_bytes >= (512ull << 20) ? 21u
-                                : table_bytes >= (128ull << 20) ? 20u : 19u;
+                                : table_bytes >= (64ull << 20) ? 21u : 19u;
     const uint32_t bb_log2 = floor_log2(d->bytes_per_bucket);
     slice_shift = slice_log2 > bb_log2 ? slice_log2 - bb_log2 : 0u;
     // buckets 0 .. bucket_cnt: a MAGIC alternate index can equal bucket_cnt (it reads the victim bytes)

@@ -108,7 +108,7 @@ typedef struct dtl_probe_params {
   void* ev_start;           /* optional hipEvent_t recorded right before the probe phase */
   void* ev_stop;            /* optional hipEvent_t recorded right after it (before compaction) */
   uint32_t slice_log2;      /* partitioned probe: log2 of the filter bytes per partition
-                               (default 19; 20 from 128 MiB, 21 from 512 MiB) */
+                               (default 19; 21 from 64 MiB) */
   uint32_t reserved;
 } dtl_probe_params;
 
