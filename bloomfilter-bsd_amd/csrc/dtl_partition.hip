@@ -42,6 +42,7 @@ namespace dtlh {
 
 constexpr int kHistThreads = 256;
 constexpr int kHistUnroll = 4;
+constexpr int kHitUnroll = 8;  // positions in flight per thread in the hit sort
 #ifndef DTL_SCATTER_KPT
 #define DTL_SCATTER_KPT 16
 #endif
@@ -280,7 +281,16 @@ __global__ __launch_bounds__(256) void k_hit_count(const uint32_t* __restrict__ 
   __syncthreads();
   uint64_t b0, b1;
   hit_slice(d_hits, &b0, &b1);
-  for (uint64_t i = b0 + tid; i < b1; i += 256) atomicAdd(&hist[hp[i] >> shift], 1u);
+  // kHitUnroll positions in flight per thread, then their LDS atomics
+  uint64_t i = b0 + tid;
+  for (; i + (kHitUnroll - 1) * 256 < b1; i += kHitUnroll * 256) {
+    uint32_t p[kHitUnroll];
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) p[u] = hp[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) atomicAdd(&hist[p[u] >> shift], 1u);
+  }
+  for (; i < b1; i += 256) atomicAdd(&hist[hp[i] >> shift], 1u);
   __syncthreads();
   uint32_t* row = hcounts + static_cast<uint64_t>(blockIdx.x) * nbuckets;
   for (uint32_t b = tid; b < nbuckets; b += 256) row[b] = hist[b];
@@ -343,7 +353,18 @@ __global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ 
   __syncthreads();
   uint64_t b0, b1;
   hit_slice(d_hits, &b0, &b1);
-  for (uint64_t i = b0 + tid; i < b1; i += 256) {
+  uint64_t i = b0 + tid;
+  for (; i + (kHitUnroll - 1) * 256 < b1; i += kHitUnroll * 256) {
+    uint32_t p[kHitUnroll];
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) p[u] = hp[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) {
+      const uint32_t b = p[u] >> shift;
+      out[base[b] + atomicAdd(&rank[b], 1u)] = p[u];
+    }
+  }
+  for (; i < b1; i += 256) {
     const uint32_t p = hp[i];
     const uint32_t b = p >> shift;
     out[base[b] + atomicAdd(&rank[b], 1u)] = p;
@@ -364,8 +385,17 @@ __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ s
   for (uint32_t i = tid; i < words32; i += 256) lbits[i] = 0u;
   __syncthreads();
   const uint32_t beg = boffsets[b], end = boffsets[b + 1];
-  for (uint32_t i = beg + tid; i < end; i += 256) {
-    const uint32_t p = sorted[i] & ((1u << shift) - 1u);
+  const uint32_t lmask = (1u << shift) - 1u;
+  uint32_t i = beg + tid;
+  for (; i + (kHitUnroll - 1) * 256 < end; i += kHitUnroll * 256) {
+    uint32_t p[kHitUnroll];
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) p[u] = sorted[i + u * 256] & lmask;
+#pragma unroll
+    for (int u = 0; u < kHitUnroll; ++u) atomicOr(&lbits[p[u] >> 5], 1u << (p[u] & 31u));
+  }
+  for (; i < end; i += 256) {
+    const uint32_t p = sorted[i] & lmask;
     atomicOr(&lbits[p >> 5], 1u << (p & 31u));
   }
   __syncthreads();
