@@ -60,10 +60,12 @@ def make_inputs(torch, log2_m, n_probe, hit_rate, seed):
     return build, probe, n_build
 
 
-def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_step=None):
+def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_step=None, warm_ms=0.0):
     """Returns (ms per step, avg ms of the probe kernel, matches).  If
     per_step is a dict it receives the median, min and max of the per-call
-    HIP-event times (ms)."""
+    HIP-event times (ms).  Warm-up runs `warmup` calls and, with warm_ms,
+    keeps calling until that much wall time has passed (GPU clocks ramp up
+    over tens of milliseconds after idle)."""
     n = probe.numel()
     out = torch.empty(n, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -75,6 +77,10 @@ def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_ste
     for _ in range(warmup):
         D.bbf_probe(desc, words, probe, 0, params=params, out=out, count=cnt)
     torch.cuda.synchronize()
+    w0 = time.time()
+    while (time.time() - w0) * 1e3 < warm_ms:
+        D.bbf_probe(desc, words, probe, 0, params=params, out=out, count=cnt)
+        torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
@@ -192,7 +198,7 @@ def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, wa
     D.bbf_build(desc, words, build)
     torch.cuda.synchronize()
     st = {}
-    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st)
+    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st, warm_ms=200.0)
     return {"config": label, "filter_bytes": int(desc.size_bytes),
             "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
             "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
@@ -206,10 +212,11 @@ def run_regimes(torch, D, steps_arg):
     random-read rate beside the two LDS-staged ones."""
     log("bench: filter-size regimes")
     steps = max(3, steps_arg // 2)
-    regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 16 KiB"),
-            regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 4 MiB"),
-            regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, 1, "u64 w8 s8 k8, 128 MiB"),
-            regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, 1,
+    warm = 3
+    regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 16 KiB"),
+            regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 4 MiB"),
+            regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 128 MiB"),
+            regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, warm,
                    "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
     for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
         lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
