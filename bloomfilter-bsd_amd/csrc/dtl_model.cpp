@@ -210,6 +210,21 @@ int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_str
     rc = DTL_EHIP;
   hipEvent_t a = nullptr, b = nullptr;
   if (rc == DTL_OK && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = DTL_EHIP;
+  // GPU clocks ramp up over tens of milliseconds after idle: probe for
+  // 200 ms before the first timed cell, so that the first configurations of
+  // the sweep are not measured at low clocks
+  if (rc == DTL_OK && nc > 0) {
+    const Cfg& C0 = M->cfgs[0];
+    dtl_bbf_desc d0;
+    rc = dtl_bbf_desc_init(&d0, C0.word_bits, std::max<uint64_t>(1ull << lmin, 3ull * C0.word_bits * C0.w), C0.k,
+                           C0.w, C0.s, C0.z);
+    const auto w0 = std::chrono::steady_clock::now();
+    while (rc == DTL_OK && std::chrono::steady_clock::now() - w0 < std::chrono::milliseconds(200)) {
+      rc = bbf_probe(&d0, filt.p, static_cast<uint32_t*>(keys.p), nkeys, 0, static_cast<uint32_t*>(pos.p),
+                     static_cast<uint64_t*>(cnt.p), stream);
+      if (rc == DTL_OK && hipStreamSynchronize(st) != hipSuccess) rc = DTL_EHIP;
+    }
+  }
   // lookup cost per (config, size): a zero filter, so every key is a
   // negative probe (the common case) and the time covers the whole call
   for (int c = 0; c < nc && rc == DTL_OK; ++c) {
