@@ -41,7 +41,7 @@
 namespace dtlh {
 
 constexpr int kHistThreads = 256;
-constexpr int kHistUnroll = 4;
+constexpr int kHistUnroll = 8;
 constexpr int kHitUnroll = 8;  // positions in flight per thread in the hit sort
 #ifndef DTL_SCATTER_KPT
 #define DTL_SCATTER_KPT 16
