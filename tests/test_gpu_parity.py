@@ -600,6 +600,31 @@ def test_large_batches_exact(ntiles, path):
     assert np.array_equal(got.astype(np.uint64), exp)
 
 
+@pytest.mark.parametrize("log2_m,log2_n", [(27, 25), (28, 23), (26, 27)])
+def test_default_path_at_break_even_sizes(log2_m, log2_n):
+    """16 MiB with 2^25 keys, 32 MiB with 2^23 and 8 MiB with 2^27 take the
+    partitioned probe by default: the default call's list equals the forced
+    direct and forced partitioned ones and the oracle's on every key."""
+    cfg = (64, 1 << log2_m, 8, 8, 8, 1)
+    f = O.Bloom(*cfg)
+    desc = D.bbf_desc(*cfg)
+    build = np.random.default_rng(log2_m).integers(0, 2**31, (1 << log2_m) // 12, dtype=np.uint64).astype(np.uint32)
+    words = gpu_build(desc, build)
+    ow = oracle_words(f, build)
+    probe = big_probe_keys(build, (1 << log2_n) - 5, log2_n)
+    kd = to_dev(probe)
+    lists = []
+    for flags in (0, D.PROBE_DIRECT, D.PROBE_PARTITIONED):
+        out, cnt = D.bbf_probe(desc, words, kd, 2, params=D.ProbeParams(0, 0, 0, 0, flags))
+        torch.cuda.synchronize()
+        lists.append(D.positions(out, cnt).astype(np.uint64))
+    import cpu_baseline as CB
+    _, _, parts = CB.bbf_probe(f, ow, probe, min(16, CB.physical_cores()))
+    exp = np.concatenate(parts).astype(np.uint64) + 2
+    for got in lists:
+        assert len(got) == len(exp) and np.array_equal(got, exp)
+
+
 def test_full_size_1gib_properties():
     cfg = (64, 1 << 33, 8, 8, 8, 1)
     desc = D.bbf_desc(*[cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]])
