@@ -375,6 +375,28 @@ def test_cuckoo_partitioned_large_batch():
     assert len(got) == len(exp) and np.array_equal(got, exp)
 
 
+def test_cuckoo_default_path_at_break_even_size():
+    """A 32 MiB table probed with 2^23 keys takes the partitioned two-pass
+    probe by default: same list as the forced direct kernel and the oracle's."""
+    import cpu_baseline as CB
+    m = (1 << 22) * 64
+    d = D.cf_desc(m, 16, 4)
+    o = O.Cuckoo(m, 16, 4)
+    keys = distinct_keys(int(0.9 * d.bucket_cnt * 4), 21)
+    buf, ins = D.cf_build_host(d, keys, seed=21)
+    assert ins == len(keys)
+    probe = big_probe_keys(keys, (1 << 23) + 3, 22)
+    dbuf = torch.from_numpy(buf).to(dev())
+    kd = to_dev(probe)
+    _, _, parts = CB.cf_probe(o, buf, probe, min(16, CB.physical_cores()))
+    exp = np.concatenate(parts).astype(np.uint64) + 1
+    for prm in (None, D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT)):
+        out, cnt = D.cf_probe(d, dbuf, kd, 1, params=prm)
+        torch.cuda.synchronize()
+        got = D.positions(out, cnt).astype(np.uint64)
+        assert len(got) == len(exp) and np.array_equal(got, exp)
+
+
 # ---------------------------------------------------------------------------
 # cuckoo GPU build: concurrent inserts give a different table than the
 # sequential insert, so parity is "valid table + same probe semantics": the
