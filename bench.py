@@ -555,7 +555,8 @@ def main():
     ap.add_argument("--log2-m", type=int, default=33)
     ap.add_argument("--log2-probe", type=int, default=28)
     ap.add_argument("--hit-rate", type=float, default=0.01)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 28,
+                    help="probe keys of the CPU baseline sample (default: the whole batch, ~26 CPU-seconds)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gather-only", action="store_true",
                     help="only run the random-sector gather microbenchmark (all access patterns)")
