@@ -242,7 +242,8 @@ int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_pro
  * lookup that picks (word size, w, s, z, k, m) from (n, tw)
  * (model/calibration.cpp:251-311, skyline_matrix.hpp:118-155,
  * optimizer.hpp:25-140) with GPU-timed calibration, measured FPRs and an FP32
- * 10-64-64-1 MLP; the calibrated table is the fallback and the judge. */
+ * 10-64-64-1 MLP; the calibrated table is the fallback and the yardstick of
+ * the MLP's regret. */
 typedef struct dtl_calib_params {
   uint32_t log2_m_min;      /* smallest calibrated filter, log2 bits (default 13) */
   uint32_t log2_m_max;      /* largest, log2 bits (default 33 = 1 GiB) */
