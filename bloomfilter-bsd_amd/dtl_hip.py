@@ -208,6 +208,19 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _dev(t, what, itemsize=None):
+    """Checks that `t` is a contiguous GPU tensor (with elements of `itemsize`
+    bytes when given): the library reads and writes device memory only, so
+    a host tensor would reach a kernel as an invalid address."""
+    if not getattr(t, "is_cuda", False):
+        raise ValueError("%s must be a tensor in GPU memory" % what)
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % what)
+    if itemsize is not None and t.element_size() != itemsize:
+        raise ValueError("%s must have %d-byte elements (got %s)" % (what, itemsize, t.dtype))
+    return t
+
+
 def new_filter(desc, device="cuda"):
     """Zeroed device buffer for a blocked Bloom filter (uint8 tensor)."""
     import torch
@@ -215,6 +228,8 @@ def new_filter(desc, device="cuda"):
 
 
 def bbf_build(desc, words, keys, stream=None):
+    _dev(words, "words")
+    _dev(keys, "keys", 4)
     _check("bbf_build", load().bbf_build(C.byref(desc), _ptr(words), _ptr(keys), keys.numel(),
                                          _stream(stream)))
 
@@ -222,6 +237,8 @@ def bbf_build(desc, words, keys, stream=None):
 def bbf_probe(desc, words, keys, match_offset=0, params=None, stream=None, out=None, count=None):
     """Returns (positions tensor of capacity n, device count tensor)."""
     import torch
+    _dev(words, "words")
+    _dev(keys, "keys", 4)
     n = keys.numel()
     if out is None:
         out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
@@ -268,8 +285,11 @@ def cf_build(desc, buf, keys, seed=0, max_kicks=0, failed=None, stream=None):
     `buf`; returns the device uint64 count of walks whose tag found no place
     (added to `failed` when given).  0 means a valid table."""
     import torch
+    _dev(buf, "table")
+    _dev(keys, "keys", 4)
     if failed is None:
         failed = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _dev(failed, "failed", 8)
     _check("cf_build", load().cf_build(C.byref(desc), _ptr(buf), _ptr(keys), keys.numel(), seed, max_kicks,
                                        _ptr(failed), _stream(stream)))
     return failed
@@ -277,6 +297,8 @@ def cf_build(desc, buf, keys, seed=0, max_kicks=0, failed=None, stream=None):
 
 def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None, params=None):
     import torch
+    _dev(buf, "table")
+    _dev(keys, "keys", 4)
     n = keys.numel()
     if out is None:
         out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)

@@ -197,3 +197,20 @@ def test_cf_build_rejects_bad_arguments():
     assert L.cf_build(ctypes.byref(d), ctypes.c_void_p(4100), fake, 10, 0, 0, cnt, None) == D.EINVAL  # 16 B align
     assert L.cf_build(ctypes.byref(d), fake, fake, 10, 0, (1 << 20) + 1, cnt, None) == D.EINVAL   # max_kicks cap
     assert L.cf_build(ctypes.byref(d), fake, None, 0, 0, 0, cnt, None) == D.OK                  # nothing to do
+
+
+def test_python_binding_rejects_host_and_wrong_width_tensors():
+    """The binding passes raw pointers to kernels, so it refuses host tensors
+    and keys that are not 4-byte words before any GPU call."""
+    torch = pytest.importorskip("torch")
+    desc = D.bbf_desc(32, 1024, 8, 1, 1)
+    host_words = torch.zeros(int(desc.size_bytes), dtype=torch.uint8)
+    with pytest.raises(ValueError):
+        D.bbf_probe(desc, host_words, torch.zeros(4, dtype=torch.int32))
+    with pytest.raises(ValueError):
+        D.bbf_build(desc, host_words, torch.zeros(4, dtype=torch.int64))
+    cd = D.cf_desc(1 << 16, 16, 4)
+    with pytest.raises(ValueError):
+        D.cf_probe(cd, torch.zeros(int(cd.buffer_bytes), dtype=torch.uint8), torch.zeros(4, dtype=torch.int32))
+    with pytest.raises(ValueError):
+        D.cf_build(cd, torch.zeros(int(cd.buffer_bytes), dtype=torch.uint8), torch.zeros(4, dtype=torch.int32))
