@@ -615,7 +615,7 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
-    const size_t part_bytes = up((nbins * 3 + 1) * 4);
+    const size_t part_bytes = up((nbins * 4 + 1) * 4);  // counts of both passes, offsets, cursors
     const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
     const size_t matrix_bytes = up(static_cast<size_t>(hit_groups) * nbuckets * 4);
     const size_t key_bytes = up(static_cast<size_t>(n) * 4);
@@ -653,8 +653,9 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     sp += bits_bytes;
     P.ptile_cnt = reinterpret_cast<uint32_t*>(sp);
     sp += cnt_r;
-    P.counts = reinterpret_cast<uint32_t*>(sp);
-    P.offsets = P.counts + nbins;
+    uint32_t* counts1 = reinterpret_cast<uint32_t*>(sp);
+    uint32_t* counts2 = counts1 + nbins;
+    P.offsets = counts2 + nbins;
     P.cursor = P.offsets + nbins + 1;
     sp += part_bytes;
     P.d_hits = reinterpret_cast<uint64_t*>(sp);
@@ -681,9 +682,15 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     P.stream = st;
     if (!ev(params ? params->ev_start : nullptr)) return DTL_EHIP;
     hipError_t he = hipSuccess;
+    // both passes' partition counts from one read of the keys (pass 1), when
+    // the two bin arrays fit in a workgroup's LDS
+    const bool one_hist = nparts <= 1024;
     for (uint32_t pass = 1; pass <= 2 && he == hipSuccess; ++pass) {
       P.part_kind = pass - 1;
       P.cf_which = pass;
+      P.counts = pass == 1 ? counts1 : counts2;
+      P.counts_next = (pass == 1 && one_hist) ? counts2 : nullptr;
+      P.counts_ready = (pass == 2 && one_hist) ? 1 : 0;
       he = partition_keys(P);
       if (he == hipSuccess) he = e->probe_part(P);
       if (he == hipSuccess) he = part_gather_matches(P, hp, pass == 1 ? nullptr : d_h1, pass == 1 ? d_h1 : d_h2);

@@ -87,6 +87,11 @@ struct BbfPartLaunch {
   uint32_t nparts;
   uint32_t scatter_threads;  // 256, 512 or 1024 (tile = 16 keys per thread)
   uint32_t* counts;       // nparts * kPartLanes bins
+  // cuckoo probe: pass 1 also fills the alternate-bucket bins (counts_next,
+  // right after counts) from the same read of the keys; pass 2 then finds
+  // its bins filled (counts_ready)
+  uint32_t* counts_next;
+  int counts_ready;
   uint32_t* offsets;      // bins + 1
   uint32_t* cursor;       // bins
   uint32_t* keys_p;       // n keys grouped by partition (later: hit positions)

@@ -145,6 +145,48 @@ __global__ __launch_bounds__(kHistThreads) void k_part_hist(const uint32_t* __re
   }
 }
 
+// Both partition functions of the cuckoo probe in one read of the keys: the
+// slice of the first bucket into counts0, of the alternate bucket into
+// counts1 (LDS: the two lane-major bin arrays one after the other).
+__global__ __launch_bounds__(kHistThreads) void k_part_hist2(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             PartFn pf, uint32_t nparts, uint32_t per_wg,
+                                                             uint32_t tile_log2, uint32_t* __restrict__ counts0,
+                                                             uint32_t* __restrict__ counts1) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nbins = nparts * kLanes;
+  for (uint32_t i = tid; i < 2 * nbins; i += kHistThreads) hist[i] = 0u;
+  __syncthreads();
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * per_wg;
+  const uint64_t end = std::min<uint64_t>(begin + per_wg, n);
+  for (uint64_t base = begin; base < end; base += kHistThreads * kHistUnroll) {
+    uint32_t key[kHistUnroll];
+    bool valid[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
+      valid[u] = i < end;
+      key[u] = load_key(keys, valid[u] ? i : begin);
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const uint64_t i = base + static_cast<uint64_t>(u) * kHistThreads + tid;
+      if (valid[u]) {
+        const uint32_t row = lane_of(i, tile_log2) * nparts;
+        atomicAdd(&hist[row + part_of<0>(key[u], pf)], 1u);
+        atomicAdd(&hist[nbins + row + part_of<1>(key[u], pf)], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nbins; i += kHistThreads) {
+    const uint32_t j = (i % kLanes) * nparts + i / kLanes;  // global bin i = partition * kLanes + lane
+    const uint32_t c0 = hist[j], c1 = hist[nbins + j];
+    if (c0) atomicAdd(&counts0[i], c0);
+    if (c1) atomicAdd(&counts1[i], c1);
+  }
+}
+
 // One workgroup: offsets[p] = sum of counts[< p], offsets[nparts] = total;
 // cursor[p] = offsets[p].
 __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* __restrict__ counts,
@@ -430,29 +472,57 @@ static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
   return hipGetLastError();
 }
 
-template <int KIND>
-static hipError_t partition_keys_kind(const BbfPartLaunch& L, const PartFn& pf) {
-  const hipStream_t st = L.stream;
-  const uint32_t nbins = L.nparts * kLanes;
-  hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(nbins)), dim3(256), 0, st, L.counts,
-                     static_cast<uint64_t>(nbins));
-  // histogram: about 8 workgroups per CU, each a multiple of 1024 keys
+// Launch shape of the histogram (about 8 workgroups per CU, each a multiple
+// of kHistThreads * kHistUnroll keys) and of the scatter.
+struct PartShape {
+  uint32_t per_wg, hist_grid, threads, tile_log2;
+};
+
+static PartShape part_shape(const BbfPartLaunch& L) {
+  PartShape S;
   const uint64_t unit = kHistThreads * kHistUnroll;
   const uint64_t want_wg = static_cast<uint64_t>(std::max(L.cu_count, 1)) * 8;
   uint64_t per_wg = (L.n + want_wg - 1) / want_wg;
   per_wg = std::max<uint64_t>(unit, (per_wg + unit - 1) / unit * unit);
-  const uint32_t hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
-  const uint32_t threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
-  const uint32_t tile_log2 = (threads == 1024 ? 10u : threads == 512 ? 9u : 8u) + kScatterKptLog2;
-  hipLaunchKernelGGL(k_part_hist<KIND>, dim3(hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n, pf,
-                     L.nparts, static_cast<uint32_t>(per_wg), tile_log2, L.counts);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.counts, nbins, L.offsets,
-                     L.cursor);
-  switch (threads) {
+  S.per_wg = static_cast<uint32_t>(per_wg);
+  S.hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
+  S.threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
+  S.tile_log2 = (S.threads == 1024 ? 10u : S.threads == 512 ? 9u : 8u) + kScatterKptLog2;
+  return S;
+}
+
+// Scan of L.counts (already histogrammed) and the scatter.
+template <int KIND>
+static hipError_t scan_and_scatter(const BbfPartLaunch& L, const PartFn& pf, const PartShape& S) {
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, L.stream, L.counts, L.nparts * kLanes,
+                     L.offsets, L.cursor);
+  switch (S.threads) {
     case 1024: return launch_scatter<1024, KIND>(L, pf);
     case 512: return launch_scatter<512, KIND>(L, pf);
     default: return launch_scatter<256, KIND>(L, pf);
   }
+}
+
+template <int KIND>
+static hipError_t partition_keys_kind(const BbfPartLaunch& L, const PartFn& pf) {
+  const hipStream_t st = L.stream;
+  const uint32_t nbins = L.nparts * kLanes;
+  const PartShape S = part_shape(L);
+  if (!L.counts_ready) {
+    const uint32_t nclear = L.counts_next ? 2 * nbins : nbins;  // counts_next follows counts
+    hipLaunchKernelGGL(k_fill_zero_u32, dim3(fill_grid(nclear)), dim3(256), 0, st, L.counts,
+                       static_cast<uint64_t>(nclear));
+    if (L.counts_next) {
+      const hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_hist2), 2 * nbins * 4u);
+      if (ae != hipSuccess) return ae;
+      hipLaunchKernelGGL(k_part_hist2, dim3(S.hist_grid), dim3(kHistThreads), 2 * nbins * 4u, st, L.keys, L.n,
+                         pf, L.nparts, S.per_wg, S.tile_log2, L.counts, L.counts_next);
+    } else {
+      hipLaunchKernelGGL(k_part_hist<KIND>, dim3(S.hist_grid), dim3(kHistThreads), nbins * 4u, st, L.keys, L.n,
+                         pf, L.nparts, S.per_wg, S.tile_log2, L.counts);
+    }
+  }
+  return scan_and_scatter<KIND>(L, pf, S);
 }
 
 hipError_t partition_keys(const BbfPartLaunch& L) {
