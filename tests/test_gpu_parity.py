@@ -375,6 +375,14 @@ def test_cuckoo_partitioned_large_batch():
     assert len(got) == len(exp) and np.array_equal(got, exp)
 
 
+def test_cuckoo_partitioned_many_partitions():
+    """More than 1024 partitions (4 KiB slices of an 8 MiB table): the two
+    passes count their partitions in separate histogram passes."""
+    d, buf, pos = check_cf((1 << 20) * 64, 16, 4, 0.9, (1 << 20) + 7, 0.3, seed=17, offset=3,
+                           params=(cf_part_params(12),))
+    assert len(pos) > 0
+
+
 def test_cuckoo_default_path_at_break_even_size():
     """A 32 MiB table probed with 2^23 keys takes the partitioned two-pass
     probe by default: same list as the forced direct kernel and the oracle's."""
