@@ -339,6 +339,22 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
     log("batched: autotuning the batch plan")
     batch_ms, counts, best, _ = time_graph_plan(torch, D, tasks, steps, tune_reps)
     filter_bytes = int(sum(t[0].size_bytes for t in tasks))
+    # Two byte counts per filter f with n keys and c matches.  Block model:
+    # 4 B per key, one 64 B block per key where the filter is read from
+    # memory or the whole filter once per 2048-key tile where it is staged in
+    # LDS (the tuned threshold), 4 B per match.  Compulsory HBM bytes: 4 B per
+    # key, min(filter bytes, 64 B per key) since a filter's words come from
+    # HBM at most once, 4 B per match.  Filters that fit the L2 or the
+    # Infinity Cache serve most block reads from there, so the block model
+    # can exceed the HBM peak; the compulsory count is the HBM floor.
+    block_bytes = 0
+    hbm_bytes = 0
+    for (desc, _, keys, _), c in zip(tasks, plain_counts):
+        nk = keys.numel()
+        staged = desc.size_bytes <= best.lds_max_bytes
+        tiles = (nk + 2047) // 2048
+        block_bytes += 4 * nk + (desc.size_bytes * tiles if staged else 64 * nk) + 4 * c
+        hbm_bytes += 4 * nk + min(desc.size_bytes, 64 * nk) + 4 * c
     del tasks
     torch.cuda.empty_cache()
     fp = plain_matches - true_hits
@@ -353,6 +369,18 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
            "counts_identical": bool(counts == plain_counts),
            "tuned": {"lds_max_bytes": best.lds_max_bytes, "block_threads": best.block_threads,
                      "chunk_tiles": best.chunk_tiles},
+           "roofline": {"bound": "hbm", "bytes_per_probe": round(hbm_bytes / n_keys, 2),
+                        "achieved": round(hbm_bytes / batch_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(hbm_bytes / batch_ms / 1e6 / HBM_PEAK_GBS, 4),
+                        "note": "compulsory HBM bytes: per filter 4 B per key + min(filter bytes, "
+                                "64 B per key) + 4 B per match",
+                        "block_model": {"bytes_per_probe": round(block_bytes / n_keys, 2),
+                                        "achieved": round(block_bytes / batch_ms / 1e6, 1),
+                                        "frac": round(block_bytes / batch_ms / 1e6 / HBM_PEAK_GBS, 4),
+                                        "note": "4 B per key + 64 B per key read from memory or the "
+                                                "filter once per 2048-key tile when staged in LDS + "
+                                                "4 B per match; above 1 where block reads hit L2 or "
+                                                "the Infinity Cache"}},
            "total_filter_cost_ms": {}}
     for t in tw_ns:
         row = {"plain": round(plain_ms + t * fp * 1e-6, 3), "batched": round(batch_ms + t * fp * 1e-6, 3)}
