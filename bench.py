@@ -604,8 +604,8 @@ def main():
                     help="run one 1 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box in this setup has a single MI355X, "
-              "so --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only (a single MI355X); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch

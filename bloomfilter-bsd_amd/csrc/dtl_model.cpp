@@ -1,6 +1,6 @@
 // dtl_model.cpp -- choosing a blocked Bloom filter configuration from (n, tw):
 // a GPU calibration sweep, an FP32 MLP trained on part of it, and the
-// exhaustive table as fallback and judge.
+// exhaustive table as fallback and yardstick.
 //
 // Objective (the reference's, cost_fn.hpp:143-166):
 //   overhead(config, m, n, tw) = t_lookup(config, m) + tw * fpr(config, m, n)
@@ -326,7 +326,7 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 // only depends on the ranking near the optimum, so that is where the fit has
 // to be tight.  kRestarts networks (different initialisations) are trained
 // and the one whose choices have the lowest mean regret on a query grid,
-// judged on the training cells only (candidates at calibrated sizes, table
+// measured on the training cells only (candidates at calibrated sizes, table
 // cost of those cells), is kept.
 constexpr int kRestarts = 4;
 

@@ -194,7 +194,7 @@ def test_learned_choice_and_persistence(small_model, tmp_path):
             c = M.config(int(rng.integers(0, M.config_count())))
             c.m = ex.m
             rand.append(M.cost(c, 1 << ln, tw) / ex.cost_ns - 1)
-    # per-query regret of the learned choice, judged by the calibrated table:
+    # per-query regret of the learned choice, measured against the calibrated table:
     # far below that of a random configuration (this small calibration grid,
     # 2^13..2^20 bits, is noisier than the full one the benchmark reports)
     assert np.mean(regrets) < 0.25 * np.mean(rand), (np.mean(regrets), np.mean(rand))
