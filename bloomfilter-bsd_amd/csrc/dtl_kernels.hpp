@@ -276,16 +276,20 @@ template <class C>
 struct SplitLane {
   static constexpr SplitSched P = C::split;
   static constexpr int NB = P.gpp * P.bpg;
+  static constexpr int NSETS = P.lanes / P.span;
   uint32_t fb[NB];
   uint32_t fd[P.gpp];
+  uint32_t sub;  // this lane's piece within its set (span > 1)
   __device__ __forceinline__ explicit SplitLane(uint32_t piece) {
+    const uint32_t set = piece / P.span;
+    sub = piece % P.span;
     static_for<NB>([&](auto jj) {
       constexpr int j = decltype(jj)::value, gl = j / P.bpg, i = j % P.bpg;
       uint32_t v = split_fold(C::sched.bbit[gl * P.bpg + i]);
-      static_for<P.lanes - 1>([&](auto pp) {
+      static_for<NSETS - 1>([&](auto pp) {
         constexpr int p = decltype(pp)::value + 1;
         constexpr uint32_t f = split_fold(C::sched.bbit[(p * P.gpp + gl) * P.bpg + i]);
-        v = piece == static_cast<uint32_t>(p) ? f : v;
+        v = set == static_cast<uint32_t>(p) ? f : v;
       });
       fb[j] = v;
     });
@@ -294,17 +298,18 @@ struct SplitLane {
       uint32_t v = 0;
       if constexpr (P.dbits[gl] != 0) {
         v = split_fold(C::sched.gdyn[gl]);
-        static_for<P.lanes - 1>([&](auto pp) {
+        static_for<NSETS - 1>([&](auto pp) {
           constexpr int p = decltype(pp)::value + 1;
           constexpr uint32_t f = split_fold(C::sched.gdyn[p * P.gpp + gl]);
-          v = piece == static_cast<uint32_t>(p) ? f : v;
+          v = set == static_cast<uint32_t>(p) ? f : v;
         });
       }
       fd[gl] = v;
     });
   }
 
-  // Test of the groups in this lane's piece of the key's block.
+  // Test of the groups (span 1) or of the groups' words (span > 1) in this
+  // lane's piece of the key's block.
   __device__ __forceinline__ bool test(uint32_t key, const uint4& piece_data) const {
     using W = typename C::word_t;
     W pw[P.wpp];
@@ -318,13 +323,20 @@ struct SplitLane {
         const uint32_t bit = P.bbase[j] + ((key * fb[j]) >> (32 - P.bbits[j]));
         mask |= one_at<W>(bit);
       });
-      W word;
-      if constexpr (P.dbits[gl] == 0) {
-        word = pw[P.lword[gl]];
+      if constexpr (P.span > 1) {
+        // d: the group's word within its set, span * wpp candidates
+        const uint32_t d = (key * fd[gl]) >> (32 - P.dbits[gl]);
+        const W word = select_word<0, P.wpp>(pw, d % P.wpp);
+        ok &= (d / P.wpp != sub) | ((word & mask) == mask);
       } else {
-        word = select_word<P.lword[gl], (1 << P.dbits[gl])>(pw, (key * fd[gl]) >> (32 - P.dbits[gl]));
+        W word;
+        if constexpr (P.dbits[gl] == 0) {
+          word = pw[P.lword[gl]];
+        } else {
+          word = select_word<P.lword[gl], (1 << P.dbits[gl])>(pw, (key * fd[gl]) >> (32 - P.dbits[gl]));
+        }
+        ok &= (word & mask) == mask;
       }
-      ok &= (word & mask) == mask;
     });
     return ok;
   }

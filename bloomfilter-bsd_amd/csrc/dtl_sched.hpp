@@ -170,25 +170,39 @@ constexpr Sched make_sched(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t 
 
 // Lane-split form of a schedule.  A block of 32 B or more is read by
 // `lanes` = block bytes / 16 lanes, one 16-byte piece each, and every lane
-// tests only the groups whose candidate words lie in its piece.  This needs
-// every group to stay inside one piece and every piece to hold the same
-// pattern of groups (same first word relative to the piece, same dynamic
-// word-offset width, same bits per group with the same widths and bases);
-// only the hash sources and shifts differ from piece to piece, and a lane
-// folds those into per-lane multipliers (split_fold) once per kernel.
-// Holds for s >= w (one group per word), for s < w with sectors of at most
-// 16 B and for zoned filters with zones of at most 16 B.
+// tests only the bits whose word lies in its piece.  The pieces fall into
+// `lanes / span` sets of `span` pieces, and every set holds the same pattern
+// of groups (same first word relative to the set, same dynamic word-offset
+// width, same bits per group with the same widths and bases); only the hash
+// sources and shifts differ from set to set, and a lane folds those into
+// per-lane multipliers (split_fold) once per kernel.
+//   span 1: every group's candidate words stay inside one piece (s >= w,
+//           sectors or zones of at most 16 B);
+//   span > 1: every group's candidate words are exactly the span pieces of
+//           its set (sectors or zones of 32 B or more); the lane of sub-piece
+//           q tests a group only when its dynamic word offset d has
+//           d / (words per piece) == q, and passes it otherwise, so the AND
+//           over the key's lanes is the block test.  Every lane of a set
+//           hashes and selects all of the set's groups, so this form is only
+//           kept for at most kSplitSpanGroups groups per set: zoned filters
+//           (one group per zone) always, sectors of 32 B or more for k/s <= 4
+//           (measured on MI355X, 2^26 probes: u32 w16 z2 k6 1.15-1.47x and
+//           u64 w16 z2 k8 1.42-1.54x over one lane per key, u64 w8 s2 k8
+//           1.04-1.27x; at 8 and 16 groups per set 8-60% slower).
 struct SplitSched {
   bool ok = false;
   int lanes = 1;           // 16-byte pieces per block
+  int span = 1;            // pieces per set
   int wpp = 0;             // words per piece
-  int gpp = 0;             // groups per piece
+  int gpp = 0;             // groups per set
   int bpg = 0;             // bits per group
-  uint8_t lword[16] = {};  // first word of local group gl, relative to its piece
+  uint8_t lword[16] = {};  // first word of local group gl, relative to its set
   uint8_t dbits[16] = {};  // dynamic word-offset width of local group gl
   uint8_t bbits[16] = {};  // width of local bit gl * bpg + i
   uint8_t bbase[16] = {};  // base of local bit gl * bpg + i
 };
+
+constexpr int kSplitSpanGroups = 4;
 
 constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w) {
   SplitSched P{};
@@ -196,16 +210,25 @@ constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w)
   if (bytes < 32 || S.ngroups == 0) return P;
   const int L = static_cast<int>(bytes / 16);
   const int wpp = static_cast<int>(16 / word_bytes);
-  if (S.ngroups % L != 0 || S.nbits % S.ngroups != 0) return P;
-  const int gpp = S.ngroups / L, bpg = S.nbits / S.ngroups;
+  const int gw = 1 << S.gdyn[0].nbits;  // candidate words of a group
+  const int span = gw > wpp ? gw / wpp : 1;
+  if (span > L || L % span != 0) return P;
+  const int sets = L / span;
+  if (S.ngroups % sets != 0 || S.nbits % S.ngroups != 0) return P;
+  const int gpp = S.ngroups / sets, bpg = S.nbits / S.ngroups;
+  if (span > 1 && gpp > kSplitSpanGroups) return P;
   for (int b = 0; b < S.nbits; ++b)
     if (S.bgroup[b] != b / bpg) return P;
   for (int g = 0; g < S.ngroups; ++g) {
-    const int piece = g / gpp, gl = g % gpp;
-    const int first = S.gword[g], last = first + (1 << S.gdyn[g].nbits) - 1;
-    if (first / wpp != piece || last / wpp != piece) return P;
-    const int lw = first - piece * wpp;
-    if (piece == 0) {
+    const int set = g / gpp, gl = g % gpp;
+    const int first = S.gword[g], words = 1 << S.gdyn[g].nbits, last = first + words - 1;
+    if (span == 1) {
+      if (first / wpp != set || last / wpp != set) return P;
+    } else if (words != span * wpp || first != set * span * wpp) {
+      return P;
+    }
+    const int lw = first - set * span * wpp;
+    if (set == 0) {
       P.lword[gl] = static_cast<uint8_t>(lw);
       P.dbits[gl] = S.gdyn[g].nbits;
     } else if (P.lword[gl] != lw || P.dbits[gl] != S.gdyn[g].nbits) {
@@ -215,7 +238,7 @@ constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w)
       const Chunk c = S.bbit[g * bpg + i];
       const int j = gl * bpg + i;
       if (c.nbits == 0) return P;
-      if (piece == 0) {
+      if (set == 0) {
         P.bbits[j] = c.nbits;
         P.bbase[j] = c.base;
       } else if (P.bbits[j] != c.nbits || P.bbase[j] != c.base) {
@@ -225,6 +248,7 @@ constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w)
   }
   P.ok = true;
   P.lanes = L;
+  P.span = span;
   P.wpp = wpp;
   P.gpp = gpp;
   P.bpg = bpg;
