@@ -178,17 +178,18 @@ constexpr Sched make_sched(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t 
 // per-lane multipliers (split_fold) once per kernel.
 //   span 1: every group's candidate words stay inside one piece (s >= w,
 //           sectors or zones of at most 16 B);
-//   span > 1: every group's candidate words are exactly the span pieces of
-//           its set (sectors or zones of 32 B or more); the lane of sub-piece
-//           q tests a group only when its dynamic word offset d has
-//           d / (words per piece) == q, and passes it otherwise, so the AND
-//           over the key's lanes is the block test.  Every lane of a set
-//           hashes and selects all of the set's groups, so this form is only
-//           kept for at most kSplitSpanGroups groups per set: zoned filters
-//           (one group per zone) always, sectors of 32 B or more for k/s <= 4
-//           (measured on MI355X, 2^26 probes: u32 w16 z2 k6 1.15-1.47x and
-//           u64 w16 z2 k8 1.42-1.54x over one lane per key, u64 w8 s2 k8
-//           1.04-1.27x; at 8 and 16 groups per set 8-60% slower).
+//   span > 1 (zoned filters with zones of 32 B or more): every zone's
+//           candidate words are exactly the span pieces of its set; the lane
+//           of sub-piece q tests the zone only when its dynamic word offset d
+//           has d / (words per piece) == q, and passes it otherwise, so the
+//           AND over the key's lanes is the block test.
+// Every lane of a set hashes and selects all of the set's groups.  A zone is
+// one group, so zoned filters always take the spanning form (measured on
+// MI355X, 2^26 probes: u32 w16 z2 k6 1.15-1.47x, u64 w16 z2 k8 1.42-1.54x
+// over one lane per key).  A sector of 32 B or more holds k/s groups; the
+// spanning form measured 1.04-1.27x at k/s <= 4 but 8-60% slower at 8 and 16,
+// and a split form that came and went with k inside one (word, w, s) family
+// is a step the learned cost model smooths over, so sectors keep span 1.
 struct SplitSched {
   bool ok = false;
   int lanes = 1;           // 16-byte pieces per block
@@ -202,9 +203,7 @@ struct SplitSched {
   uint8_t bbase[16] = {};  // base of local bit gl * bpg + i
 };
 
-constexpr int kSplitSpanGroups = 4;
-
-constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w) {
+constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w, bool zoned) {
   SplitSched P{};
   const uint32_t bytes = w * word_bytes;
   if (bytes < 32 || S.ngroups == 0) return P;
@@ -216,7 +215,7 @@ constexpr SplitSched make_split(const Sched& S, uint32_t word_bytes, uint32_t w)
   const int sets = L / span;
   if (S.ngroups % sets != 0 || S.nbits % S.ngroups != 0) return P;
   const int gpp = S.ngroups / sets, bpg = S.nbits / S.ngroups;
-  if (span > 1 && gpp > kSplitSpanGroups) return P;
+  if (span > 1 && !zoned) return P;
   for (int b = 0; b < S.nbits; ++b)
     if (S.bgroup[b] != b / bpg) return P;
   for (int g = 0; g < S.ngroups; ++g) {
@@ -291,7 +290,7 @@ struct BbfCfg {
   static constexpr uint32_t z = Zc;
   static constexpr uint32_t k = Kc;
   static constexpr Sched sched = make_sched(word_bits, Wc, Sc, Zc, Kc);
-  static constexpr SplitSched split = make_split(sched, sizeof(W), Wc);
+  static constexpr SplitSched split = make_split(sched, sizeof(W), Wc, Zc > 1);
   static_assert(Zc > 1 ? valid_zoned(word_bits, Wc, Zc, Kc) : valid_sgew_sltw(word_bits, Wc, Sc, Kc),
                 "unsupported configuration");
 };

@@ -79,24 +79,23 @@ def test_every_reference_configuration_is_compiled():
 
 def test_lane_split_classification():
     """The lane-split kernel exists for blocks of at least 32 B, with lanes
-    per key = block bytes / 16: sectors and zones of at most 16 B keep each
-    group inside one piece; larger zones (one group each) and larger sectors
-    with k/s <= 4 groups span whole pieces, each lane testing the words in
-    its own piece; larger sectors with more groups have no split form."""
+    per key = block bytes / 16: for zoned filters always (zones of 32 B and
+    up span whole pieces, each lane testing the words in its own piece), for
+    the others where every word a bit can land in lies inside one 16-byte
+    piece: s >= w (one sector per word or finer) or sectors of at most 16 B."""
     L = D.load()
     for wb, w, s, z, k in valid_configs():
         block = w * wb // 8
         if z > 1:
             ok = True
         else:
-            local = s >= w or (w // s) * wb // 8 <= 16
-            ok = local or k // s <= 4
+            ok = s >= w or (w // s) * wb // 8 <= 16
         want = block // 16 if block >= 32 and ok else 0
         assert L.dtl_bbf_split_lanes(wb, w, s, z, k) == want, (wb, w, s, z, k)
     assert L.dtl_bbf_split_lanes(64, 8, 8, 1, 8) == 4      # the cache-sectorized benchmark filter
-    assert L.dtl_bbf_split_lanes(64, 8, 2, 1, 8) == 4      # 32 B sectors span two pieces
+    assert L.dtl_bbf_split_lanes(64, 8, 2, 1, 8) == 0      # 32 B sectors cross pieces
     assert L.dtl_bbf_split_lanes(32, 16, 16, 2, 6) == 4    # 32 B zones (the semi-join's filter)
-    assert L.dtl_bbf_split_lanes(32, 16, 1, 1, 16) == 0    # one 64 B sector, 16 groups: one lane
+    assert L.dtl_bbf_split_lanes(64, 16, 16, 2, 8) == 8    # 64 B zones of a 128 B block
     assert L.dtl_bbf_split_lanes(32, 4, 4, 1, 4) == 0      # 16 B blocks: one lane already
     assert L.dtl_bbf_split_lanes(32, 1, 8, 1, 8) == 0      # not a configuration
 

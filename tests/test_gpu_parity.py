@@ -186,7 +186,7 @@ def test_every_configuration_global_paths():
             torch.cuda.synchronize()
             assert np.array_equal(D.positions(out, cnt), exp), (cfg, params.flags)
         n_split += L.dtl_bbf_split_lanes(wb, w, s, z, k) > 0
-    assert n_split == 143  # the configurations with a lane-split kernel (host test pins the rule)
+    assert n_split == 107  # the configurations with a lane-split kernel (host test pins the rule)
 
 
 @pytest.mark.parametrize("cfg", [
@@ -197,10 +197,9 @@ def test_every_configuration_global_paths():
     (32, 10 * 400000, 12, 16, 16, 4),  # zones of 16 B
     (64, 1 << 24, 6, 4, 4, 2),        # u64 zones of 16 B
     (32, 10 * 300007, 6, 16, 16, 2),  # zones of 32 B span two pieces, MAGIC
-    (64, 1 << 24, 8, 8, 2, 1),        # sectors of 32 B span two pieces
-    (32, 1 << 24, 4, 16, 1, 1),       # one 64 B sector: every bit's word spans all 4 pieces
+    (64, 1 << 24, 8, 8, 8, 2),        # u64 zones of 32 B: 2 sets of 2 pieces
     (64, 10 * 250007, 8, 16, 16, 2),  # 128 B blocks, zones of 64 B: 2 sets of 4 pieces, MAGIC
-    (64, 1 << 25, 8, 16, 4, 1),       # 128 B blocks, sectors of 32 B
+    (32, 1 << 23, 4, 8, 8, 2),        # 32 B blocks, one zone of 16 B per piece
 ])
 def test_lane_split_partitioned_and_direct(cfg):
     """The lane-split kernel over grouped keys (partitioned probe) and over
