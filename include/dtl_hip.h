@@ -312,8 +312,10 @@ int dtl_bbf_supported(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, ui
 /* Number of compiled blocked Bloom filter configurations. */
 int dtl_bbf_config_count(void);
 /* Lanes per key (2, 4 or 8) of the lane-split probe kernel of a configuration
- * (DTL_PROBE_LANE_SPLIT), 0 if it has none (blocks under 32 B, or bits whose
- * word range crosses a 16-byte piece) or is not compiled.  No counterpart in
+ * (DTL_PROBE_LANE_SPLIT), 0 if it has none (blocks under 32 B, or a sectorized
+ * filter whose sectors cross 16-byte pieces; zoned filters with zones of 32 B
+ * or more have one, each lane testing the zone words in its own piece) or is
+ * not compiled.  No counterpart in
  * the reference (its SIMD kernels have no such choice). */
 int dtl_bbf_split_lanes(uint32_t word_bits, uint32_t w, uint32_t s, uint32_t z, uint32_t k);
 int dtl_version(void);
