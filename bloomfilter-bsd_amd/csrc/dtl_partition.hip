@@ -23,7 +23,7 @@
 //   6. reordering of the matches (only matches move, about 1-2% of keys):
 //      scan + k_expand_gather list the original positions of the matches,
 //      k_hit_count / k_hit_colscan / k_part_scan / k_hit_place bucket them by
-//      position (2^16 positions per bucket or more), and k_hit_bits builds
+//      position (2^18 positions per bucket or more), and k_hit_bits builds
 //      each bucket's piece of the key-order bitmap in LDS and writes it with
 //      its tile counts.
 // The order of keys inside a partition depends on atomic timing, but the
