@@ -229,20 +229,31 @@ def run_regimes(torch, D, steps_arg):
 FIXED_SEMI_JOIN_CFG = (32, 6, 16, 16, 2)   # (word_bits, k, w, s, z): u32 words, w=16, z=2, k=6
 
 
-def semi_join_inputs(torch, n_filters, seed, log2_build=(10, 22), log2_probe=(12, 24), hit=(0.001, 0.30)):
-    """Keys of the batched semi-join workload: n_filters partitions with build
-    sizes log-uniform in 2^10..2^22 keys and probe batches log-uniform in
-    2^12..2^24 keys at hit rates uniform in [0.1%, 30%]; seeded.  Build keys
-    are regenerated on demand (build_keys) so that only probe keys stay
-    resident."""
+def semi_join_sizes(n_filters, seed, log2_build=(10, 22), log2_probe=(12, 24), hit=(0.001, 0.30)):
+    """Per-partition sizes of the batched semi-join workload, seeded: build
+    sizes log-uniform in 2^10..2^22 keys, probe batches log-uniform in
+    2^12..2^24 keys, hit rates uniform in [0.1%, 30%].  Returns a list of
+    (n_build, n_probe, hit_rate)."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    meta, probes = [], []
-    for f in range(n_filters):
+    out = []
+    for _ in range(n_filters):
         nb = int(2 ** rng.uniform(*log2_build))
         npb = int(2 ** rng.uniform(*log2_probe))
         h = float(rng.uniform(*hit))
+        out.append((nb, npb, h))
+    return out
+
+
+def semi_join_inputs(torch, n_filters, seed):
+    """Keys of the batched semi-join workload (sizes from semi_join_sizes):
+    probe keys distinct and known absent, round(rate x n_probe) of them
+    replaced at random positions by random build keys.  Build keys are
+    regenerated on demand (build_keys) so that only probe keys stay
+    resident."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    meta, probes = [], []
+    for f, (nb, npb, h) in enumerate(semi_join_sizes(n_filters, seed)):
         build = build_keys(torch, f, nb)
         probe = distinct_keys(torch, npb, 3 * f + 100001, 1)
         hits = int(round(h * npb))

@@ -621,7 +621,8 @@ uint64_t orc_cf_entries(const orc_cf* f, const void* buf, uint32_t* bucket, uint
 /* ---------------------------------------------------------------------- */
 /* FPR model (blocked_bloomfilter/math.hpp:17-145, fpr.hpp:11-60)           */
 /* The Poisson pmf is evaluated as exp(i ln(l) - l - lgamma(i+1)) instead of */
-/* Boost.Math: parity unpinned.                                            */
+/* Boost.Math's pdf; tests/test_oracle_fpr.py pins it against the exact     */
+/* l^i e^-l / i! and every formula below against closed forms.             */
 /* ---------------------------------------------------------------------- */
 static double poisson_pmf(double lambda, uint64_t i) {
   if (i == 0) return exp(-lambda);
@@ -658,6 +659,8 @@ double orc_fpr_sectorized(uint64_t m, uint64_t n, double k, uint64_t B, uint64_t
   return f;
 }
 
+double orc_poisson_pmf(double lambda, uint64_t i) { return poisson_pmf(lambda, i); }
+
 static double p_cache(uint64_t s, uint64_t S, uint64_t B, double k, uint64_t i) {
   const double k_per_s = k / (double)s;
   double sum = 0.0;
@@ -666,6 +669,10 @@ static double p_cache(uint64_t s, uint64_t S, uint64_t B, double k, uint64_t i) 
     sum += poisson_pmf(ev, j) * orc_fpr_bloom(S, j, k_per_s);
   }
   return pow(sum, (double)s);
+}
+
+double orc_fpr_p_cache(uint64_t s, uint64_t S, uint64_t B, double k, uint64_t i) { /* math.hpp:99-113 */
+  return p_cache(s, S, B, k, i);
 }
 
 double orc_fpr_zoned(uint64_t m, uint64_t n, double k, uint64_t B, uint64_t S, double z, int self_collisions, double eps) {

@@ -25,8 +25,14 @@
  *                             negatives, SWAR vs slot compare
  *                             (test/cuckoofilter_test.cpp:10-66), mt19937
  *                             known answer (C++ standard, 10000th draw)
- *   FPR model (fpr_*)         parity unpinned: the reference evaluates the
- *                             Poisson pmf with Boost.Math, which is absent.
+ *   FPR model (fpr_*)         the Poisson pmf against the exact l^i e^-l / i!;
+ *                             blocked and sectorized mixtures against their
+ *                             closed forms through the Poisson generating
+ *                             function; the self-collision adjustment against
+ *                             enumerated expected distinct bits; the zoned
+ *                             inner term against its closed form where the
+ *                             truncation is immaterial (tests/test_oracle_fpr.py).
+ *                             The reference's Boost.Math pdf itself is absent.
  */
 #ifndef DTL_ORACLE_H
 #define DTL_ORACLE_H
@@ -153,7 +159,9 @@ void orc_cf_key_candidates(const orc_cf* f, const uint32_t* keys, uint64_t n, ui
 uint64_t orc_cf_entries(const orc_cf* f, const void* buf, uint32_t* bucket, uint32_t* tag);
 
 /* ---- FPR model (blocked_bloomfilter/math.hpp, fpr.hpp) ------------------ */
+double orc_poisson_pmf(double lambda, uint64_t i);
 double orc_fpr_bloom(uint64_t m, uint64_t n, double k);
+double orc_fpr_p_cache(uint64_t s, uint64_t S, uint64_t B, double k, uint64_t i);
 double orc_fpr_blocked(uint64_t m, uint64_t n, double k, uint64_t B, int self_collisions, double eps);
 double orc_fpr_sectorized(uint64_t m, uint64_t n, double k, uint64_t B, uint64_t S, int self_collisions, double eps);
 double orc_fpr_zoned(uint64_t m, uint64_t n, double k, uint64_t B, uint64_t S, double z, int self_collisions, double eps);

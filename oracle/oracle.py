@@ -120,6 +120,18 @@ def lib():
         L.orc_fpr_config.restype = C.c_double
         L.orc_fpr_config.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint32, C.c_uint32]
+        L.orc_poisson_pmf.restype = C.c_double
+        L.orc_poisson_pmf.argtypes = [C.c_double, C.c_uint64]
+        L.orc_fpr_blocked.restype = C.c_double
+        L.orc_fpr_blocked.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_int, C.c_double]
+        L.orc_fpr_sectorized.restype = C.c_double
+        L.orc_fpr_sectorized.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64,
+                                         C.c_int, C.c_double]
+        L.orc_fpr_zoned.restype = C.c_double
+        L.orc_fpr_zoned.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64,
+                                    C.c_double, C.c_int, C.c_double]
+        L.orc_fpr_p_cache.restype = C.c_double
+        L.orc_fpr_p_cache.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64]
         L.orc_fpr_cuckoo.restype = C.c_double
         L.orc_fpr_cuckoo.argtypes = [C.c_uint32, C.c_uint32, C.c_double]
         L.orc_fpr_bloom.restype = C.c_double
