@@ -475,6 +475,33 @@ def run_learned(torch, D, train_fraction=0.25, samples=4000000, epochs=16, model
     return res, M
 
 
+def cuckoo_build_keys(n_build):
+    """n_build distinct uint32 keys below 2^31 for the cuckoo workload (numpy;
+    the same seeded bijection as distinct_keys, seed 9)."""
+    import numpy as np
+    M = (1 << 31) - 1
+    x = (np.arange(n_build, dtype=np.uint64) + np.uint64(9 << 20)) & np.uint64(M)
+    x = (x * np.uint64(0x5851F42D)) & np.uint64(M)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x2C1B3C6D)) & np.uint64(M)
+    x ^= x >> np.uint64(13)
+    x = (x * np.uint64(0x297A2D39)) & np.uint64(M)
+    return x.astype(np.uint32)
+
+
+def cuckoo_probe_keys(torch, build_dev, n, hit):
+    """n probe keys on the GPU, distinct and absent (bit 31 set), with
+    round(hit x n) of them replaced at random positions by build keys;
+    returns (keys, planted hits)."""
+    g = torch.Generator(device="cuda").manual_seed(5)
+    probe = distinct_keys(torch, n, 77, 1)
+    hits = int(round(hit * n))
+    idx = torch.randperm(n, device="cuda", generator=g)[:hits]
+    probe[idx] = build_dev[torch.randint(0, build_dev.numel(), (hits,), device="cuda", generator=g)]
+    torch.cuda.synchronize()
+    return probe, hits
+
+
 def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, load=0.95):
     """Cuckoo filter, 16-bit tags, 4 slots per bucket, 2^26 buckets (512 MiB +
     victim cache), loaded to 95% with distinct uniform keys by the host build
@@ -484,15 +511,7 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
     desc = D.cf_desc((1 << log2_buckets) * 64, 16, 4)
     n_build = int(load * (1 << log2_buckets) * 4)
     log("cuckoo: generating %d distinct keys" % n_build)
-    M = (1 << 31) - 1
-    x = (np.arange(n_build, dtype=np.uint64) + np.uint64(9 << 20)) & np.uint64(M)
-    x = (x * np.uint64(0x5851F42D)) & np.uint64(M)
-    x ^= x >> np.uint64(16)
-    x = (x * np.uint64(0x2C1B3C6D)) & np.uint64(M)
-    x ^= x >> np.uint64(13)
-    x = (x * np.uint64(0x297A2D39)) & np.uint64(M)
-    keys = x.astype(np.uint32)
-    del x
+    keys = cuckoo_build_keys(n_build)
     log("cuckoo: host build")
     t0 = time.time()
     buf, inserted = D.cf_build_host(desc, keys, seed=1)
@@ -528,12 +547,7 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
     del gbuf, found_out
     for hit in (0.01, 0.5):
         n = 1 << 28
-        g = torch.Generator(device="cuda").manual_seed(5)
-        probe = distinct_keys(torch, n, 77, 1)
-        hits = int(round(hit * n))
-        idx = torch.randperm(n, device="cuda", generator=g)[:hits]
-        probe[idx] = build_dev[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
-        torch.cuda.synchronize()
+        probe, hits = cuckoo_probe_keys(torch, build_dev, n, hit)
 
         def timed(params):
             for _ in range(warmup):

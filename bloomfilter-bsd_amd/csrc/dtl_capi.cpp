@@ -154,21 +154,32 @@ int current_device(int* dev, int* cu_count, uint32_t* max_lds) {
 struct Scratch {
   void* ptr = nullptr;
   size_t bytes = 0;
+  // set once a call on this stream was captured into a graph: the graph
+  // holds ptr, so the buffer may neither move nor be freed from then on
+  bool captured = false;
 };
 
+static std::mutex g_scratch_mu;
+static std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+
+static bool stream_capturing(hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 // Scratch device memory per (device, stream), grown on demand.  Growth
-// synchronizes the stream before freeing the old buffer; a stream that is
-// being captured into a graph must have been warmed up with a call of at
-// least the same size.
+// synchronizes the stream before freeing the old buffer.  A call captured
+// into a graph pins the buffer: the graph's kernels hold its address, so a
+// later call that needs more than it holds fails with DTL_ERUNTIME (before
+// any work is enqueued) instead of freeing memory a replay would use, until
+// dtl_scratch_release() is called for the stream.  Capturing therefore
+// needs a warm-up call of at least the same size on the stream.
 static int get_scratch(int dev, hipStream_t stream, size_t bytes, void** out) {
-  static std::mutex mu;
-  static std::map<std::pair<int, hipStream_t>, Scratch> cache;
-  std::lock_guard<std::mutex> g(mu);
-  Scratch& s = cache[{dev, stream}];
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  Scratch& s = g_scratch[{dev, stream}];
+  const bool capturing = stream_capturing(stream);
   if (s.bytes < bytes) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
-      return DTL_ERUNTIME;
+    if (capturing || s.captured) return DTL_ERUNTIME;
     if (s.ptr) {
       if (hipStreamSynchronize(stream) != hipSuccess) return DTL_EHIP;
       (void)hipFree(s.ptr);
@@ -182,8 +193,38 @@ static int get_scratch(int dev, hipStream_t stream, size_t bytes, void** out) {
     }
     s.bytes = want;
   }
+  if (capturing) s.captured = true;
   *out = s.ptr;
   return DTL_OK;
+}
+
+// Frees the scratch of (current device, stream) and clears its pin.
+static int release_scratch(hipStream_t stream) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DTL_ENODEVICE;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch.find({dev, stream});
+  if (it == g_scratch.end()) return DTL_OK;
+  if (stream_capturing(stream)) return DTL_ERUNTIME;
+  if (it->second.ptr) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return DTL_EHIP;
+    (void)hipFree(it->second.ptr);
+  }
+  g_scratch.erase(it);
+  return DTL_OK;
+}
+
+static size_t scratch_bytes(hipStream_t stream, int* captured) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch.find({dev, stream});
+  if (it == g_scratch.end()) {
+    if (captured) *captured = 0;
+    return 0;
+  }
+  if (captured) *captured = it->second.captured ? 1 : 0;
+  return it->second.bytes;
 }
 
 static thread_local char g_last_error[256] = "";
@@ -251,7 +292,15 @@ using namespace dtlh;
 
 extern "C" {
 
-int dtl_version(void) { return 100; }
+int dtl_version(void) { return 101; }
+
+int dtl_scratch_release(dtl_stream_t stream) {
+  return release_scratch(static_cast<hipStream_t>(stream));
+}
+
+uint64_t dtl_scratch_bytes(dtl_stream_t stream, int* captured) {
+  return scratch_bytes(static_cast<hipStream_t>(stream), captured);
+}
 
 const char* dtl_last_error(void) { return g_last_error; }
 

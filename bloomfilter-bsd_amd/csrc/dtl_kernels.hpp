@@ -463,6 +463,21 @@ constexpr bool lds_swizzled() {
   return true;
 }
 
+// Word i of type W at an address holding 16-byte pieces (the staged LDS
+// filter): read through a byte pointer with its alignment stated, so that the
+// access is one ds_read of the word's width and no object is read through a
+// type it was not written as.
+template <class W>
+struct WordsAt {
+  const char* p;
+  __device__ __forceinline__ W operator[](uint32_t i) const {
+    const void* a = __builtin_assume_aligned(p + static_cast<size_t>(i) * sizeof(W), sizeof(W));
+    W v;
+    __builtin_memcpy(&v, a, sizeof(W));
+    return v;
+  }
+};
+
 template <class C>
 struct LdsLayout {
   static constexpr bool kSwz = lds_swizzled<C>();
@@ -472,28 +487,42 @@ struct LdsLayout {
   __device__ __forceinline__ static uint32_t slot(uint32_t b, uint32_t p) {
     return b * P + ((p + (b >> R)) & (P - 1));
   }
+  // The filter's bytes are copied as 16-byte pieces, and a 4-byte tail as
+  // 4-byte words written through a byte pointer (filters are multiples of 4
+  // bytes); readers take words with WordsAt or whole pieces.
   __device__ __forceinline__ static void stage(uint4* __restrict__ lds, const void* filter, uint32_t filter_bytes,
                                                uint32_t tid, uint32_t nthreads) {
     const uint32_t n16 = filter_bytes / 16;
-    const uint4* src = reinterpret_cast<const uint4*>(filter);
+    const uint4* src = static_cast<const uint4*>(filter);
     if constexpr (kSwz) {
       for (uint32_t i = tid; i < n16; i += nthreads) lds[slot(i / P, i % P)] = src[i];
     } else {
       for (uint32_t i = tid; i < n16; i += nthreads) lds[i] = src[i];
-      uint32_t* dst32 = reinterpret_cast<uint32_t*>(lds);
-      const uint32_t* src32 = reinterpret_cast<const uint32_t*>(filter);
-      for (uint32_t i = n16 * 4 + tid; i < filter_bytes / 4; i += nthreads) dst32[i] = src32[i];
+      char* dst = reinterpret_cast<char*>(lds);
+      const char* srcb = static_cast<const char*>(filter);
+      for (uint32_t i = n16 * 4 + tid; i < filter_bytes / 4; i += nthreads) {
+        uint32_t v;
+        __builtin_memcpy(&v, __builtin_assume_aligned(srcb + 4 * i, 4), 4);
+        __builtin_memcpy(__builtin_assume_aligned(dst + 4 * i, 4), &v, 4);
+      }
     }
   }
   __device__ __forceinline__ static bool test(uint32_t key, uint32_t b, const uint4* __restrict__ lds) {
     using W = typename C::word_t;
     if constexpr (kSwz) {
-      uint4 piece[P];
+      // whole pieces back in logical order, then the register test (swizzled
+      // layouts have no hash-chosen word index)
+      W blk[C::w];
+      constexpr uint32_t WPP = 16 / sizeof(W);
 #pragma unroll
-      for (uint32_t p = 0; p < P; ++p) piece[p] = lds[slot(b, p)];
-      return test_ptr<C>(key, reinterpret_cast<const W*>(piece));
+      for (uint32_t p = 0; p < P; ++p) {
+        const uint4 v = lds[slot(b, p)];
+        __builtin_memcpy(&blk[p * WPP], &v, 16);
+      }
+      return test_regs<C>(key, blk);
     } else {
-      return test_ptr<C>(key, reinterpret_cast<const W*>(lds) + static_cast<uint32_t>(b * C::w));
+      const WordsAt<W> words{reinterpret_cast<const char*>(lds) + static_cast<size_t>(b) * C::w * sizeof(W)};
+      return test_ptr<C>(key, words);
     }
   }
 };

@@ -4,7 +4,7 @@
 // reference's exhaustive calibration + skyline lookup
 // (model/calibration.cpp:251-311, tool/calibration/src/skyline_matrix_builder.cpp).
 //
-// Parameter layout (floats): W1[64][8] b1[64] W2[64][64] b2[64] W3[64] b3.
+// Parameter layout (floats): W1[64][10] b1[64] W2[64][64] b2[64] W3[64] b3.
 // Training: each workgroup takes 64 samples of the minibatch, keeps the
 // activations in LDS, forms its share of the gradient with small LDS matrix
 // products (dW2 = dH2^T H1 etc.) and adds it to the global gradient with
@@ -211,6 +211,8 @@ struct dtl_mlp {
 
 extern "C" {
 
+int dtl_mlp_input_count(void) { return kIn; }
+
 int dtl_mlp_param_count(void) { return kParams; }
 
 int dtl_mlp_create(uint32_t seed, dtl_mlp** out) {
@@ -263,7 +265,7 @@ int dtl_mlp_set_params(dtl_mlp* m, const float* h_params) {
              : DTL_EHIP;
 }
 
-// Trains on n samples (d_x: n x 8 features, d_t: n targets, d_w: n sample
+// Trains on n samples (d_x: n x kIn = 10 features, d_t: n targets, d_w: n sample
 // weights or null; device memory) with Adam, minibatches of `batch` rows,
 // `epochs` passes; each epoch visits the rows in the order (a * j + c) mod n
 // for a fresh (a coprime to n, c).

@@ -85,7 +85,7 @@ EXPORTS = [
     "dtl_model_fpr", "dtl_model_config_count", "dtl_model_get_config", "dtl_model_calibration_seconds",
     "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
     "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
-    "dtl_mlp_set_params",
+    "dtl_mlp_set_params", "dtl_mlp_input_count", "dtl_scratch_release", "dtl_scratch_bytes",
 ]
 
 _lib = None
@@ -148,6 +148,10 @@ def load(path=LIB_PATH):
     L.dtl_model_destroy.argtypes = [vp]
     L.dtl_model_destroy.restype = None
     L.dtl_mlp_param_count.argtypes = []
+    L.dtl_mlp_input_count.argtypes = []
+    L.dtl_scratch_release.argtypes = [vp]
+    L.dtl_scratch_bytes.argtypes = [vp, C.POINTER(C.c_int)]
+    L.dtl_scratch_bytes.restype = u64
     L.dtl_mlp_create.argtypes = [u32, C.POINTER(vp)]
     L.dtl_mlp_destroy.argtypes = [vp]
     L.dtl_mlp_destroy.restype = None
@@ -158,6 +162,7 @@ def load(path=LIB_PATH):
     L.dtl_mlp_set_params.argtypes = [vp, C.POINTER(C.c_float)]
     for name in ("dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_config_count",
                  "dtl_model_get_config", "dtl_model_save", "dtl_model_load", "dtl_mlp_param_count",
+                 "dtl_mlp_input_count", "dtl_scratch_release",
                  "dtl_mlp_create", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient",
                  "dtl_mlp_get_params", "dtl_mlp_set_params"):
         getattr(L, name).restype = i32
@@ -208,17 +213,38 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
-def _dev(t, what, itemsize=None):
+def _dev(t, what, itemsize=None, min_bytes=None, device=None, dtype=None):
     """Checks that `t` is a contiguous GPU tensor (with elements of `itemsize`
-    bytes when given): the library reads and writes device memory only, so
-    a host tensor would reach a kernel as an invalid address."""
+    bytes, of at least `min_bytes` bytes, on `device`, of `dtype`, when
+    given): the library reads and writes device memory only, so a host or
+    undersized tensor would reach a kernel as an invalid address.  Raises
+    ValueError before any call into the library."""
     if not getattr(t, "is_cuda", False):
         raise ValueError("%s must be a tensor in GPU memory" % what)
     if not t.is_contiguous():
         raise ValueError("%s must be contiguous" % what)
     if itemsize is not None and t.element_size() != itemsize:
         raise ValueError("%s must have %d-byte elements (got %s)" % (what, itemsize, t.dtype))
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError("%s must be %s (got %s)" % (what, dtype, t.dtype))
+    if min_bytes is not None and t.numel() * t.element_size() < min_bytes:
+        raise ValueError("%s holds %d bytes, needs %d" % (what, t.numel() * t.element_size(), min_bytes))
+    if device is not None and t.device != device:
+        raise ValueError("%s is on %s, the other operands on %s" % (what, t.device, device))
     return t
+
+
+def _outputs(out, count, n, device):
+    """The position buffer (>= max(n, 1) 4-byte entries) and the device
+    uint64 count of a probe, allocated when not given."""
+    import torch
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=torch.int32, device=device)
+    if count is None:
+        count = torch.zeros(1, dtype=torch.int64, device=device)
+    _dev(out, "out", 4, min_bytes=4 * max(n, 1), device=device)
+    _dev(count, "count", 8, min_bytes=8, device=device)
+    return out, count
 
 
 def new_filter(desc, device="cuda"):
@@ -228,22 +254,18 @@ def new_filter(desc, device="cuda"):
 
 
 def bbf_build(desc, words, keys, stream=None):
-    _dev(words, "words")
-    _dev(keys, "keys", 4)
+    _dev(words, "words", min_bytes=int(desc.size_bytes))
+    _dev(keys, "keys", 4, device=words.device)
     _check("bbf_build", load().bbf_build(C.byref(desc), _ptr(words), _ptr(keys), keys.numel(),
                                          _stream(stream)))
 
 
 def bbf_probe(desc, words, keys, match_offset=0, params=None, stream=None, out=None, count=None):
     """Returns (positions tensor of capacity n, device count tensor)."""
-    import torch
-    _dev(words, "words")
-    _dev(keys, "keys", 4)
+    _dev(words, "words", min_bytes=int(desc.size_bytes))
+    _dev(keys, "keys", 4, device=words.device)
     n = keys.numel()
-    if out is None:
-        out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
-    if count is None:
-        count = torch.zeros(1, dtype=torch.int64, device=keys.device)
+    out, count = _outputs(out, count, n, keys.device)
     L = load()
     if params is None:
         rc = L.bbf_probe(C.byref(desc), _ptr(words), _ptr(keys), n, match_offset, _ptr(out),
@@ -285,25 +307,21 @@ def cf_build(desc, buf, keys, seed=0, max_kicks=0, failed=None, stream=None):
     `buf`; returns the device uint64 count of walks whose tag found no place
     (added to `failed` when given).  0 means a valid table."""
     import torch
-    _dev(buf, "table")
-    _dev(keys, "keys", 4)
+    _dev(buf, "table", min_bytes=int(desc.buffer_bytes))
+    _dev(keys, "keys", 4, device=buf.device)
     if failed is None:
-        failed = torch.zeros(1, dtype=torch.int64, device="cuda")
-    _dev(failed, "failed", 8)
+        failed = torch.zeros(1, dtype=torch.int64, device=buf.device)
+    _dev(failed, "failed", 8, min_bytes=8, device=buf.device)
     _check("cf_build", load().cf_build(C.byref(desc), _ptr(buf), _ptr(keys), keys.numel(), seed, max_kicks,
                                        _ptr(failed), _stream(stream)))
     return failed
 
 
 def cf_probe(desc, buf, keys, match_offset=0, stream=None, out=None, count=None, params=None):
-    import torch
-    _dev(buf, "table")
-    _dev(keys, "keys", 4)
+    _dev(buf, "table", min_bytes=int(desc.buffer_bytes))
+    _dev(keys, "keys", 4, device=buf.device)
     n = keys.numel()
-    if out is None:
-        out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
-    if count is None:
-        count = torch.zeros(1, dtype=torch.int64, device=keys.device)
+    out, count = _outputs(out, count, n, keys.device)
     L = load()
     if params is None:
         rc = L.cf_probe(C.byref(desc), _ptr(buf), _ptr(keys), n, match_offset, _ptr(out), _ptr(count),
@@ -367,19 +385,33 @@ class BatchPlan:
             pass
 
 
+def scratch_release(stream=None):
+    """Frees the probe scratch of the stream (dtl_scratch_release)."""
+    _check("dtl_scratch_release", load().dtl_scratch_release(_stream(stream)))
+
+
+def scratch_bytes(stream=None):
+    """(bytes, pinned) of the probe scratch held for the stream."""
+    cap = C.c_int(0)
+    b = load().dtl_scratch_bytes(_stream(stream), C.byref(cap))
+    return int(b), bool(cap.value)
+
+
 def gather_bench(buf, mode, n_reads, sink, stream=None):
     _check("dtl_gather_bench", load().dtl_gather_bench(_ptr(buf), buf.numel() * buf.element_size(), mode,
                                                        n_reads, _ptr(sink), _stream(stream)))
 
 
 class MLP:
-    """The library's FP32 10-64-64-1 MLP (dtl_mlp.hip)."""
+    """The library's FP32 MLP (dtl_mlp.hip): input_count (10) inputs, two
+    hidden layers of 64, one output."""
 
     def __init__(self, seed=0):
         self.L = load()
         self.h = C.c_void_p()
         _check("dtl_mlp_create", self.L.dtl_mlp_create(seed, C.byref(self.h)))
         self.n_params = self.L.dtl_mlp_param_count()
+        self.n_inputs = self.L.dtl_mlp_input_count()
 
     def params(self):
         import numpy as np
@@ -392,14 +424,33 @@ class MLP:
         p = np.ascontiguousarray(p, dtype=np.float32)
         _check("dtl_mlp_set_params", self.L.dtl_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
 
+    def _rows(self, x, t=None, w=None):
+        """Checks the feature matrix (n x input_count float32 on the GPU) and
+        the per-row targets and weights before any launch."""
+        import torch
+        _dev(x, "x", dtype=torch.float32)
+        if x.dim() != 2 or x.shape[1] != self.n_inputs:
+            raise ValueError("x must be n x %d (got %s)" % (self.n_inputs, tuple(x.shape)))
+        n = x.shape[0]
+        for v, what in ((t, "t"), (w, "w")):
+            if v is not None:
+                _dev(v, what, dtype=torch.float32, device=x.device)
+                if v.numel() != n:
+                    raise ValueError("%s must hold %d values (got %d)" % (what, n, v.numel()))
+        return n
+
     def predict(self, x):
         import torch
-        y = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
-        _check("dtl_mlp_predict", self.L.dtl_mlp_predict(self.h, _ptr(x), x.shape[0], _ptr(y), _stream(None)))
+        n = self._rows(x)
+        y = torch.empty(n, dtype=torch.float32, device=x.device)
+        _check("dtl_mlp_predict", self.L.dtl_mlp_predict(self.h, _ptr(x), n, _ptr(y), _stream(None)))
         return y
 
     def gradient(self, x, t, w=None):
         import torch
+        if t is None:
+            raise ValueError("t is required")
+        self._rows(x, t, w)
         g = torch.empty(self.n_params, dtype=torch.float32, device=x.device)
         _check("dtl_mlp_gradient", self.L.dtl_mlp_gradient(self.h, _ptr(x), _ptr(t),
                                                            _ptr(w) if w is not None else None,
@@ -407,6 +458,9 @@ class MLP:
         return g
 
     def train(self, x, t, epochs, batch, lr, seed=0, w=None):
+        if t is None:
+            raise ValueError("t is required")
+        self._rows(x, t, w)
         _check("dtl_mlp_train", self.L.dtl_mlp_train(self.h, _ptr(x), _ptr(t), _ptr(w) if w is not None else None,
                                                      x.shape[0], epochs, batch, lr, seed, _stream(None)))
 

@@ -29,6 +29,15 @@
  *     filter (the reference's selection vector).  d_pos must hold n entries.
  *     *d_count is a device uint64_t written by the call.
  *   - No exceptions cross the ABI: every function returns a dtl_status.
+ *   - bbf_probe / cf_probe keep library-owned scratch per (device, stream),
+ *     grown on demand.  Once a call on a stream has been captured into a
+ *     hipGraph, that stream's scratch is pinned (the graph holds its
+ *     address): a later call on the stream that needs more scratch returns
+ *     DTL_ERUNTIME, enqueues nothing and frees nothing, so warm a stream up
+ *     with the largest call before capturing it.  Replay a graph on the
+ *     stream it was captured from (its replays and the stream's eager calls
+ *     then share the scratch in stream order), and release the scratch with
+ *     dtl_scratch_release only after every such graph is destroyed.
  */
 #ifndef DTL_HIP_H
 #define DTL_HIP_H
@@ -277,13 +286,18 @@ int dtl_model_save(const dtl_cost_model* model, const char* path);
 int dtl_model_load(const char* path, dtl_cost_model** model);
 void dtl_model_destroy(dtl_cost_model* model);
 
-/* The MLP itself (FP32, parameters W1[64][8] b1[64] W2[64][64] b2[64] W3[64] b3). */
+/* The MLP itself: FP32, dtl_mlp_input_count() inputs (10), two hidden layers
+ * of 64 (ReLU), one output; parameters W1[64][IN] b1[64] W2[64][64] b2[64]
+ * W3[64] b3 with IN = dtl_mlp_input_count(). */
 typedef struct dtl_mlp dtl_mlp;
+int dtl_mlp_input_count(void);
 int dtl_mlp_param_count(void);
 int dtl_mlp_create(uint32_t seed, dtl_mlp** mlp);
 void dtl_mlp_destroy(dtl_mlp* mlp);
-/* Adam on 0.5 * mean(w * (f(x) - t)^2); d_x: n x 8 features, d_t: n targets,
- * d_w: n sample weights or NULL (all 1); device memory. */
+/* Adam on 0.5 * mean(w * (f(x) - t)^2); d_x: n rows of dtl_mlp_input_count()
+ * floats (row-major, n x 10), d_t: n targets, d_w: n sample weights or NULL
+ * (all 1); device memory.  dtl_mlp_predict and dtl_mlp_gradient read d_x the
+ * same way; a shorter buffer is read past its end. */
 int dtl_mlp_train(dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream);
 int dtl_mlp_predict(const dtl_mlp* mlp, const float* d_x, uint32_t n, float* d_y, dtl_stream_t stream);
@@ -302,6 +316,16 @@ int dtl_mlp_set_params(dtl_mlp* mlp, const float* h_params);
  * device uint64_t scratch word. */
 int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
                      uint64_t* d_sink, dtl_stream_t stream);
+
+/* ---- scratch ------------------------------------------------------------- */
+/* Frees the probe scratch of (current device, stream) after synchronizing the
+ * stream, and clears its pin (see Conventions).  Call it only once no graph
+ * that captured probe calls on the stream will be replayed.  No reference
+ * counterpart (the reference's probes allocate nothing). */
+int dtl_scratch_release(dtl_stream_t stream);
+/* Bytes of probe scratch held for (current device, stream); *captured (may
+ * be NULL) receives 1 if the scratch is pinned by a captured call. */
+uint64_t dtl_scratch_bytes(dtl_stream_t stream, int* captured);
 
 /* ---- misc ---------------------------------------------------------------- */
 const char* dtl_status_string(int status);

@@ -29,6 +29,8 @@ def lib():
             getattr(L, fn).argtypes = [C.POINTER(desc), C.c_void_p, C.c_void_p, C.c_uint64, C.c_int,
                                        C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)]
+        L.cpub_bbf_build.restype = None
+        L.cpub_bbf_build.argtypes = [C.POINTER(O.BBF), C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
         _lib = L
     return _lib
 
@@ -75,3 +77,12 @@ def bbf_simd_available(bloom):
 
 def cf_simd_available(cuckoo):
     return bool(lib().cpub_cf_simd_available(C.byref(cuckoo.d)))
+
+
+def bbf_build(bloom, keys, threads):
+    """The oracle's filter for `keys`, built on `threads` host threads (block
+    ranges per thread; equals bloom.insert on a zeroed filter)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    words = bloom.new_words()
+    lib().cpub_bbf_build(C.byref(bloom.d), words.ctypes.data, keys.ctypes.data, len(keys), threads)
+    return words

@@ -187,6 +187,28 @@ int cpub_cf_simd_available(const orc_cf* f) {
   return have_avx512() && f->tag_bits == 16 && f->tags_per_bucket == 4 && f->addr.mode == ORC_POW2;
 }
 
+// Builds the filter from keys[0..n) with `threads` threads (a checker for
+// full-size GPU builds, not a baseline): thread t owns the blocks
+// [t * B / T, (t + 1) * B / T), scans every key and inserts, with the scalar
+// oracle's orc_bbf_insert, those whose block it owns.  A key only touches
+// words of its own block, so no two threads write one word, and the result is
+// the sequential build's (OR is order-independent).
+void cpub_bbf_build(const orc_bbf* f, void* words, const uint32_t* keys, uint64_t n, int threads) {
+  threads = std::max(1, threads);
+  const uint64_t nb = f->addr.block_cnt;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    const uint64_t lo = nb * t / threads, hi = nb * (t + 1) / threads;
+    pool.emplace_back([=] {
+      for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t b = orc_addr_block_idx(&f->addr, orc_hash(0, keys[i]));
+        if (b >= lo && b < hi) orc_bbf_insert(f, words, keys[i]);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
 // Probes keys[0..n) with `threads` threads; thread t writes the positions of
 // its slice (ascending, offset + index) to outs[t] and its count to counts[t].
 // Returns the total count and the wall time of the parallel region.

@@ -350,6 +350,41 @@ int orc_bbf_key_bits(const orc_bbf* f, uint32_t key, uint64_t* word_idx, uint32_
   return (int)p.n;
 }
 
+/* Large-filter checking without a dense copy of the filter on the host.
+ * Every function below runs the same key_pairs / block addressing as
+ * orc_bbf_insert and orc_bbf_contains. */
+
+/* (word index, bit) pairs of every key of a batch, in key order; returns
+ * their number (at most 64 per key).  OR-ing 1 << bit into word index over
+ * all pairs gives exactly the words orc_bbf_batch_insert sets. */
+uint64_t orc_bbf_batch_key_bits(const orc_bbf* f, const uint32_t* keys, uint64_t n, uint64_t* word_idx,
+                                uint32_t* bit_idx) {
+  uint64_t c = 0;
+  for (uint64_t i = 0; i < n; i++) c += (uint64_t)orc_bbf_key_bits(f, keys[i], word_idx + c, bit_idx + c);
+  return c;
+}
+
+/* Block index of every key (block_addressing_logic.hpp:104-110, :205-210). */
+void orc_bbf_batch_blocks(const orc_bbf* f, const uint32_t* keys, uint64_t n, uint32_t* blocks) {
+  for (uint64_t i = 0; i < n; i++) blocks[i] = orc_addr_block_idx(&f->addr, orc_hash(0, keys[i]));
+}
+
+/* batch_contains where `blocks` holds, for key i, the w words of the block
+ * orc_bbf_batch_blocks names for it (n x w words): the same list as
+ * orc_bbf_batch_contains on the whole filter. */
+uint64_t orc_bbf_batch_contains_blocks(const orc_bbf* f, const void* blocks, const uint32_t* keys, uint64_t n,
+                                       uint32_t* match_pos, uint32_t match_offset) {
+  uint64_t c = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    pairs_t p;
+    key_pairs(f, keys[i], &p, 0);
+    int found = 1;
+    for (uint32_t j = 0; j < p.n; j++) found &= test_bit(f, blocks, i * f->w + p.word[j], p.bit[j]);
+    if (found) match_pos[c++] = (uint32_t)i + match_offset;
+  }
+  return c;
+}
+
 int orc_bbf_has_k8_quirk(const orc_bbf* f) {
   /* The scalar and SIMD schedules differ for some key iff they differ for a
    * key whose hashes are all non-zero in every consumed chunk; checking a few

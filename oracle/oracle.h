@@ -121,6 +121,11 @@ uint64_t orc_bbf_batch_contains_mixed(const orc_bbf* f, const void* words,
                                       uint32_t vector_len, uint32_t align_bytes,
                                       uint32_t key_addr_mod);
 /* 1 if the config hits the k=8 word-block write-back quirk for some key. */
+uint64_t orc_bbf_batch_key_bits(const orc_bbf* f, const uint32_t* keys, uint64_t n, uint64_t* word_idx,
+                                uint32_t* bit_idx);
+void orc_bbf_batch_blocks(const orc_bbf* f, const uint32_t* keys, uint64_t n, uint32_t* blocks);
+uint64_t orc_bbf_batch_contains_blocks(const orc_bbf* f, const void* blocks, const uint32_t* keys, uint64_t n,
+                                       uint32_t* match_pos, uint32_t match_offset);
 int orc_bbf_has_k8_quirk(const orc_bbf* f);
 
 /* ---- cuckoo filter (cuckoofilter_logic.hpp, cuckoofilter_table.hpp) ---- */

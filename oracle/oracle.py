@@ -87,6 +87,13 @@ def lib():
                                                    C.c_uint32]
         L.orc_bbf_key_bits.restype = C.c_int
         L.orc_bbf_key_bits.argtypes = [C.POINTER(BBF), C.c_uint32, _u64p, _u32p]
+        L.orc_bbf_batch_key_bits.restype = C.c_uint64
+        L.orc_bbf_batch_key_bits.argtypes = [C.POINTER(BBF), _u32p, C.c_uint64, _u64p, _u32p]
+        L.orc_bbf_batch_blocks.restype = None
+        L.orc_bbf_batch_blocks.argtypes = [C.POINTER(BBF), _u32p, C.c_uint64, _u32p]
+        L.orc_bbf_batch_contains_blocks.restype = C.c_uint64
+        L.orc_bbf_batch_contains_blocks.argtypes = [C.POINTER(BBF), C.c_void_p, _u32p, C.c_uint64, _u32p,
+                                                    C.c_uint32]
         L.orc_bbf_has_k8_quirk.restype = C.c_int
         L.orc_bbf_has_k8_quirk.argtypes = [C.POINTER(BBF)]
         L.orc_mt_seed.restype = None
@@ -209,6 +216,41 @@ class Bloom:
         bi = np.zeros(64, dtype=np.uint32)
         n = lib().orc_bbf_key_bits(C.byref(self.d), int(key), wi, bi)
         return list(zip(wi[:n].tolist(), bi[:n].tolist()))
+
+    def sparse_words(self, keys):
+        """(sorted word indexes, their words) of the filter orc_bbf_batch_insert
+        builds from `keys`: every other word is zero."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        cap = max(len(keys) * 64, 1)
+        wi = np.zeros(cap, dtype=np.uint64)
+        bi = np.zeros(cap, dtype=np.uint32)
+        c = lib().orc_bbf_batch_key_bits(C.byref(self.d), keys, len(keys), wi, bi)
+        wi, bi = wi[:c], bi[:c]
+        order = np.argsort(wi, kind="stable")
+        wi, bi = wi[order], bi[order]
+        masks = np.left_shift(np.ones(c, dtype=np.uint64), bi.astype(np.uint64))
+        if c == 0:
+            return wi, masks.astype(self.dtype)
+        starts = np.concatenate([[0], np.nonzero(wi[1:] != wi[:-1])[0] + 1])
+        vals = np.bitwise_or.reduceat(masks, starts)
+        return wi[starts], vals.astype(self.dtype)
+
+    def blocks_of(self, keys):
+        """Block index of every key."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        out = np.zeros(max(len(keys), 1), dtype=np.uint32)
+        lib().orc_bbf_batch_blocks(C.byref(self.d), keys, len(keys), out)
+        return out[: len(keys)]
+
+    def batch_contains_blocks(self, blocks, keys, match_offset=0):
+        """batch_contains where blocks[i] (w words) is key i's block."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        blocks = np.ascontiguousarray(blocks, dtype=self.dtype)
+        assert blocks.size == len(keys) * self.d.w
+        out = np.zeros(max(len(keys), 1), dtype=np.uint32)
+        c = lib().orc_bbf_batch_contains_blocks(C.byref(self.d), blocks.ctypes.data, keys, len(keys), out,
+                                                match_offset)
+        return out[:c].copy()
 
     def has_k8_quirk(self):
         return bool(lib().orc_bbf_has_k8_quirk(C.byref(self.d)))

@@ -217,3 +217,89 @@ def test_python_binding_rejects_host_and_wrong_width_tensors():
         D.cf_probe(cd, torch.zeros(int(cd.buffer_bytes), dtype=torch.uint8), torch.zeros(4, dtype=torch.int32))
     with pytest.raises(ValueError):
         D.cf_build(cd, torch.zeros(int(cd.buffer_bytes), dtype=torch.uint8), torch.zeros(4, dtype=torch.int32))
+
+
+class _FakeDeviceTensor:
+    """Looks like a contiguous GPU tensor to the binding's argument checks;
+    its data pointer is never used because every case below must be refused
+    before the library is called."""
+
+    def __init__(self, numel, itemsize, dtype=None, device="cuda:0", shape=None):
+        import torch
+        self.is_cuda = True
+        self._numel = numel
+        self._itemsize = itemsize
+        self.dtype = dtype if dtype is not None else {1: torch.uint8, 4: torch.int32, 8: torch.int64}[itemsize]
+        self.device = torch.device(device)
+        self.shape = shape if shape is not None else (numel,)
+
+    def is_contiguous(self):
+        return True
+
+    def element_size(self):
+        return self._itemsize
+
+    def numel(self):
+        return self._numel
+
+    def dim(self):
+        return len(self.shape)
+
+    def data_ptr(self):
+        raise AssertionError("the binding called into the library with an invalid argument")
+
+
+def test_python_binding_rejects_undersized_and_mismatched_buffers():
+    """Undersized filters, tables, position buffers and counts, and operands
+    on different devices, raise ValueError without a library call."""
+    pytest.importorskip("torch")
+    desc = D.bbf_desc(64, 1 << 20, 8, 8, 8, 1)
+    F = _FakeDeviceTensor
+    keys = F(1000, 4)
+    with pytest.raises(ValueError, match="words"):
+        D.bbf_probe(desc, F(int(desc.size_bytes) - 1, 1), keys)
+    with pytest.raises(ValueError, match="words"):
+        D.bbf_build(desc, F(int(desc.size_bytes) // 2, 1), keys)
+    words = F(int(desc.size_bytes), 1)
+    with pytest.raises(ValueError, match="out"):
+        D.bbf_probe(desc, words, keys, out=F(999, 4), count=F(1, 8))
+    with pytest.raises(ValueError, match="count"):
+        D.bbf_probe(desc, words, keys, out=F(1000, 4), count=F(1, 4))
+    with pytest.raises(ValueError, match="keys"):
+        D.bbf_probe(desc, words, F(1000, 4, device="cuda:1"), out=F(1000, 4), count=F(1, 8))
+    with pytest.raises(ValueError, match="out"):
+        D.bbf_probe(desc, words, keys, out=F(1000, 4, device="cuda:1"), count=F(1, 8))
+    cd = D.cf_desc(1 << 20, 16, 4)
+    with pytest.raises(ValueError, match="table"):
+        D.cf_probe(cd, F(int(cd.buffer_bytes) - 8, 1), keys)
+    with pytest.raises(ValueError, match="table"):
+        D.cf_build(cd, F(int(cd.table_words) * 4, 1), keys)
+    with pytest.raises(ValueError, match="out"):
+        D.cf_probe(cd, F(int(cd.buffer_bytes), 1), keys, out=F(10, 4), count=F(1, 8))
+
+
+def test_mlp_input_width_is_queried_and_documented():
+    """The MLP reads rows of dtl_mlp_input_count() = 10 floats; the header
+    says so, the parameter count follows from it, and the binding refuses
+    feature matrices of any other width before a launch."""
+    import torch
+    L = D.load()
+    n_in = L.dtl_mlp_input_count()
+    assert n_in == 10
+    assert L.dtl_mlp_param_count() == 64 * n_in + 64 + 64 * 64 + 64 + 64 + 1
+    header = open(os.path.join(ROOT, "include", "dtl_hip.h")).read()
+    assert "n x 10" in header and "W1[64][IN]" in header
+    assert "n x 8" not in header and "W1[64][8]" not in header
+    mlp = D.MLP.__new__(D.MLP)      # the argument checks only; creating one needs a GPU
+    mlp.L, mlp.h, mlp.n_inputs, mlp.n_params = L, None, n_in, L.dtl_mlp_param_count()
+    F = _FakeDeviceTensor
+    for bad in (F(64 * 8, 4, torch.float32, shape=(64, 8)), F(64 * 11, 4, torch.float32, shape=(64, 11)),
+                F(640, 4, torch.float32, shape=(640,)), F(640, 8, torch.float64, shape=(64, 10))):
+        with pytest.raises(ValueError, match="x"):
+            mlp.predict(bad)
+    x = F(640, 4, torch.float32, shape=(64, 10))
+    with pytest.raises(ValueError, match="t"):
+        mlp.gradient(x, F(63, 4, torch.float32))
+    with pytest.raises(ValueError, match="w"):
+        mlp.train(x, F(64, 4, torch.float32), 1, 64, 1e-3, w=F(64, 2, torch.float16))
+    mlp.h = None

@@ -46,3 +46,37 @@ def test_generic_configs_fall_back_to_scalar():
     probe = np.concatenate([build, rng.integers(0, 2**32, 9000, dtype=np.uint64).astype(np.uint32)])
     total, secs, lists = CB.bbf_probe(f, words, probe, 4)
     assert np.array_equal(np.concatenate(lists), f.batch_contains(words, probe))
+
+
+@pytest.mark.parametrize("cfg", [(64, 1 << 22, 8, 8, 8, 1), (32, 10 * 30011, 6, 16, 16, 2),
+                                 (32, 1 << 20, 4, 1, 1, 1), (64, 3 << 20, 16, 16, 2, 1)])
+def test_threaded_build_matches_sequential(cfg):
+    """The block-sharded threaded build (a checker for full-size GPU builds)
+    equals the oracle's sequential insert."""
+    f = O.Bloom(*cfg)
+    keys = np.random.default_rng(4).integers(0, 2**32, 70000, dtype=np.uint64).astype(np.uint32)
+    seq = f.new_words()
+    f.insert(seq, keys)
+    for threads in (1, 3, 8):
+        assert np.array_equal(CB.bbf_build(f, keys, threads), seq)
+
+
+@pytest.mark.parametrize("cfg", [(64, 1 << 26, 8, 8, 8, 1), (32, (1 << 27) - 5, 6, 16, 16, 2),
+                                 (64, 1 << 20, 5, 2, 1, 1), (32, 1 << 30, 16, 8, 2, 1)])
+def test_sparse_and_block_checkers_match_dense_oracle(cfg):
+    """The oracle's sparse build (touched words only) and its batch_contains
+    on gathered blocks agree with the dense build and batch_contains."""
+    f = O.Bloom(*cfg)
+    rng = np.random.default_rng(8)
+    keys = rng.integers(0, 2**31, 5000, dtype=np.uint64).astype(np.uint32)
+    dense = f.new_words()
+    f.insert(dense, keys)
+    idx, vals = f.sparse_words(keys)
+    assert np.count_nonzero(dense) == len(idx)
+    assert np.array_equal(dense[idx.astype(np.int64)], vals)
+    probe = np.concatenate([keys[:2000], rng.integers(0, 2**32, 20000, dtype=np.uint64).astype(np.uint32)])
+    rng.shuffle(probe)
+    blocks = f.blocks_of(probe).astype(np.int64)
+    w = f.d.w
+    gathered = dense.reshape(-1, w)[blocks]
+    assert np.array_equal(f.batch_contains_blocks(gathered, probe, 9), f.batch_contains(dense, probe, 9))

@@ -2,8 +2,8 @@
 
 Bar: bit-exact.  Filter words are byte-compared with the oracle's build of the
 same keys; match lists must equal the oracle's batch_contains element for
-element, including match_offset.  Full-size filters (1 GiB) are checked
-through size-independent properties plus exact agreement on a sample.
+element, including match_offset.  The benchmark workloads at full size are
+checked in tests/test_gpu_fullsize.py.
 """
 import numpy as np
 import pytest
@@ -659,50 +659,6 @@ def test_default_path_at_break_even_sizes(log2_m, log2_n):
         assert len(got) == len(exp) and np.array_equal(got, exp)
 
 
-def test_full_size_1gib_properties():
-    cfg = (64, 1 << 33, 8, 8, 8, 1)
-    desc = D.bbf_desc(*[cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]])
-    f = O.Bloom(*cfg)
-    n_build = (1 << 33) // 12
-    import bench
-    g = torch.Generator(device="cuda").manual_seed(1)
-    build = bench.distinct_keys(torch, n_build, 5, 0)
-    words = D.new_filter(desc)
-    D.bbf_build(desc, words, build)
-    n_probe = 1 << 28
-    probe = bench.distinct_keys(torch, n_probe, 6, 1)
-    hits = int(0.01 * n_probe)
-    idx = torch.randperm(n_probe, device="cuda", generator=g)[:hits]
-    probe[idx] = build[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
-    out, cnt = D.bbf_probe(desc, words, probe, 0)
-    torch.cuda.synchronize()
-    c = int(cnt.item())
-    pos = out[:c]
-    assert c >= hits
-    assert bool((pos[1:] > pos[:-1]).all())  # strictly ascending
-    # every planted hit is reported (no false negatives)
-    isin = torch.zeros(n_probe, dtype=torch.bool, device="cuda")
-    isin[pos.long()] = True
-    assert bool(isin[idx].all())
-    # At this size (2^24 blocks chosen by 24 bits of a bijective 32-bit
-    # multiplicative hash) only 8 hash bits of entropy are left inside a
-    # block, and the reference's filter has a false-positive rate of ~0.03%,
-    # far below the 0.42% of the random-hash formula (fpr.hpp:11-60); the
-    # oracle measures the same on the CPU.  So the rate is bounded here and the
-    # exact check is the prefix comparison below.
-    fpr = (c - hits) / (n_probe - hits)
-    model = O.fpr_config(desc.m_actual, n_build, 8, 8, 8, 8, 1)
-    assert 0.01 * model < fpr < model, (fpr, model)
-    # exact agreement with the oracle's batch_contains on the first 2^21 keys
-    host_words = words.cpu().numpy().view(np.uint64)
-    pre = 1 << 21
-    exp = f.batch_contains(host_words, probe[:pre].cpu().numpy().view(np.uint32), 0)
-    got = pos[pos < pre].cpu().numpy().view(np.uint32)
-    assert np.array_equal(got, exp)
-    bs = build[torch.randint(0, n_build, (20000,), device="cuda", generator=g)].cpu().numpy().view(np.uint32)
-    assert len(f.batch_contains(host_words, bs, 0)) == len(bs)
-
-
 def test_batched_autotune_keeps_results():
     rng = np.random.default_rng(123)
     tasks, expect = [], []
@@ -787,3 +743,53 @@ def test_gather_bench_modes_run(mode, log2_bytes):
     sink = torch.zeros(1, dtype=torch.int64, device="cuda")
     D.gather_bench(buf, mode, (1 << 20) + 5, sink)
     torch.cuda.synchronize()
+
+
+def test_scratch_growth_after_capture_is_refused():
+    """A probe captured into a hipGraph pins its stream's scratch: a later,
+    larger call on that stream returns DTL_ERUNTIME without enqueuing work or
+    freeing the buffer, replays keep giving the eager result, and after
+    dtl_scratch_release (once the graph is gone) the larger call runs."""
+    cfg = (64, 1 << 25, 8, 8, 8, 1)
+    f = O.Bloom(*cfg)
+    desc = D.bbf_desc(*cfg)
+    build = np.random.default_rng(31).integers(0, 2**31, 100000, dtype=np.uint64).astype(np.uint32)
+    words = gpu_build(desc, build)
+    ow = oracle_words(f, build)
+    small = mixed_keys(build, 50000, 0.2, 32)
+    large = mixed_keys(build, 1 << 23, 0.2, 33)   # needs more than the 1 MiB minimum scratch
+    ks, kl = to_dev(small), to_dev(large)
+    s = torch.cuda.Stream()
+    out = torch.empty(1 << 23, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    with torch.cuda.stream(s):
+        D.scratch_release(s)
+        D.bbf_probe(desc, words, ks, 0, stream=s, out=out, count=cnt)   # warm-up sizes the scratch
+    torch.cuda.synchronize()
+    before, pinned = D.scratch_bytes(s)
+    assert before > 0 and not pinned
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        D.bbf_probe(desc, words, ks, 0, stream=s, out=out, count=cnt)
+    assert D.scratch_bytes(s) == (before, True)
+    exp_small = f.batch_contains(ow, small, 0)
+    with torch.cuda.stream(s):
+        with pytest.raises(D.DtlError) as ei:
+            D.bbf_probe(desc, words, kl, 0, stream=s, out=out, count=cnt)
+    assert ei.value.status == D.ERUNTIME
+    assert D.scratch_bytes(s) == (before, True)
+    for _ in range(3):
+        cnt.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(D.positions(out, cnt), exp_small)
+    del g
+    torch.cuda.synchronize()
+    D.scratch_release(s)
+    assert D.scratch_bytes(s) == (0, False)
+    with torch.cuda.stream(s):
+        D.bbf_probe(desc, words, kl, 0, stream=s, out=out, count=cnt)
+    torch.cuda.synchronize()
+    assert np.array_equal(D.positions(out, cnt), f.batch_contains(ow, large, 0))
+    D.scratch_release(s)
