@@ -233,6 +233,8 @@ void set_last_error(hipError_t e) {
   std::snprintf(g_last_error, sizeof(g_last_error), "%s (%d)", hipGetErrorString(e), static_cast<int>(e));
 }
 
+void set_last_error_text(const char* msg) { std::snprintf(g_last_error, sizeof(g_last_error), "%s", msg); }
+
 static int hip_status(hipError_t e) {
   if (e == hipSuccess) return DTL_OK;
   set_last_error(e);

@@ -239,8 +239,9 @@ inline hipError_t allow_dynamic_lds(const void* fn, uint32_t bytes) {
 // Writes n distinct keys (seeded bijection of [0, 2^31), bit 31 = high_bit).
 hipError_t gen_keys(uint32_t* out, uint64_t n, uint32_t seed, uint32_t high_bit, hipStream_t st);
 
-// Records the text of a failed HIP call for dtl_last_error().
+// Records the text of a failed HIP call (or a message) for dtl_last_error().
 void set_last_error(hipError_t e);
+void set_last_error_text(const char* msg);
 
 // Device info of the current device (CU count, LDS bytes per workgroup).
 int current_device(int* dev, int* cu_count, uint32_t* max_lds);

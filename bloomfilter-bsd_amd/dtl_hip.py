@@ -86,6 +86,7 @@ EXPORTS = [
     "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
     "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
     "dtl_mlp_set_params", "dtl_mlp_input_count", "dtl_scratch_release", "dtl_scratch_bytes",
+    "bbf_probe_noalloc", "dtl_noalloc_max_keys",
 ]
 
 _lib = None
@@ -107,6 +108,9 @@ def load(path=LIB_PATH):
     L.bbf_build.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, vp]
     L.bbf_probe.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, vp]
     L.bbf_probe_ex.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
+    L.bbf_probe_noalloc.argtypes = [C.POINTER(BbfDesc), vp, vp, u64, u32, vp, vp, vp]
+    L.dtl_noalloc_max_keys.argtypes = []
+    L.dtl_noalloc_max_keys.restype = u64
     L.cf_probe.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, vp]
     L.cf_probe_ex.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, vp, vp, C.POINTER(ProbeParams), vp]
     L.cf_build_host.argtypes = [C.POINTER(CfDesc), vp, vp, u64, u32, C.POINTER(C.c_uint64)]
@@ -166,7 +170,7 @@ def load(path=LIB_PATH):
                  "dtl_mlp_create", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient",
                  "dtl_mlp_get_params", "dtl_mlp_set_params"):
         getattr(L, name).restype = i32
-    for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex",
+    for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "bbf_probe_noalloc",
                  "cf_probe", "cf_probe_ex", "cf_build_host", "cf_build", "bbf_batch_plan_create", "bbf_batch_plan_run",
                  "bbf_probe_batched", "dtl_bbf_supported", "dtl_bbf_config_count", "dtl_bbf_split_lanes",
                  "dtl_version",
@@ -274,6 +278,18 @@ def bbf_probe(desc, words, keys, match_offset=0, params=None, stream=None, out=N
         rc = L.bbf_probe_ex(C.byref(desc), _ptr(words), _ptr(keys), n, match_offset, _ptr(out),
                             _ptr(count), C.byref(params), _stream(stream))
     _check("bbf_probe", rc)
+    return out, count
+
+
+def bbf_probe_noalloc(desc, words, keys, match_offset=0, stream=None, out=None, count=None):
+    """bbf_probe_noalloc: the probe with the library's fixed per-GPU scratch
+    (no device allocation in the library); returns (positions, count)."""
+    _dev(words, "words", min_bytes=int(desc.size_bytes))
+    _dev(keys, "keys", 4, device=words.device)
+    n = keys.numel()
+    out, count = _outputs(out, count, n, keys.device)
+    _check("bbf_probe_noalloc", load().bbf_probe_noalloc(C.byref(desc), _ptr(words), _ptr(keys), n, match_offset,
+                                                         _ptr(out), _ptr(count), _stream(stream)))
     return out, count
 
 

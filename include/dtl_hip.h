@@ -157,6 +157,22 @@ int bbf_build(const dtl_bbf_desc* d, void* d_words, const uint32_t* d_keys, uint
 int bbf_probe(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
               uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
 
+/* bbf_probe without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same list as
+ * bbf_probe; it never takes the partitioned probe (large filters probed with
+ * large batches run at the direct kernel's rate).  Its match bitmap and tile
+ * counts live in one fixed device array of 33.1 MB that each GPU the library
+ * runs on holds until the process ends; n above dtl_noalloc_max_keys()
+ * (2^28) returns DTL_EINVAL.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams, or unordered host
+ * threads) race; calls ordered on one stream do not.  It calls no hipMalloc
+ * or hipFree. */
+int bbf_probe_noalloc(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
+                      uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
+/* Largest n bbf_probe_noalloc accepts (2^28). */
+uint64_t dtl_noalloc_max_keys(void);
+
 /* bbf_probe with explicit launch parameters (NULL = defaults). */
 int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_keys, uint64_t n,
                  uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count,
