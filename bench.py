@@ -1,15 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the filter probe path on one MI355X.
 
-Headline workload (one "step" = one bbf_probe call over one batch):
-  cache-sectorized blocked Bloom filter, 64-bit words, w=8 words (512-bit
-  blocks), s=8 sectors, k=8, m = 2^33 bits (1 GiB), built from m/12 uniform
-  uint32 keys; probe batch of 2^28 uint32 keys at a 1% hit rate; output is the
-  ascending match-position list.  Inputs are resident in HBM before timing.
-
-Prints one JSON line (see the `result` dict) with the whole-job probe rate,
-the roofline of the dominant kernel (the probe kernel, timed with HIP events
-on the stream it runs on) and a CPU baseline measured on this host.
+Prints one JSON line.  The headline ("value", one "step" = one bbf_probe call
+over one batch): the cache-sectorized blocked Bloom filter -- 64-bit words,
+w=8 words (512-bit blocks), s=8 sectors, k=8, m = 2^33 bits (1 GiB), built
+from m/12 distinct uint32 keys -- probed with 2^28 uint32 keys at a 1% hit
+rate; output is the ascending match-position list.  Inputs are resident in
+HBM before timing.  The same line carries every other workload of the metric,
+each with its roofline, a CPU baseline timed on this host's cores and an
+element-wise comparison of its list with the CPU baseline's (outside the
+timed region):
+  regimes            the same filter at 16 KiB, 4 MiB and 128 MiB (2^28 keys), and
+                     the plain register-blocked filter (u32, w=1, k=4, 128 KiB,
+                     2^16 keys; 2^24 probes at 10% hits);
+  cuckoo             16-bit tags, 4 slots, 2^26 buckets at 95% load (host build),
+                     2^28 probes at 1% and 50% hits;
+  learned_config     the calibration sweep, the MLP and its regret against the
+                     exhaustive choice;
+  batched_semi_join  4096 zoned partition filters: plain per-filter calls of the
+                     fixed configuration vs one tuned, graph-captured batch plan,
+                     and the learned vs exhaustive configurations in batch plans,
+                     total filter cost at tw = 1, 10, 100 and 1000 ns per tuple.
 """
 import argparse
 import json
@@ -22,6 +33,8 @@ sys.path.insert(0, os.path.join(ROOT, "bloomfilter-bsd_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0
+GPU_CLOCK_GHZ = 2.4     # MI355X peak engine clock: converts tw in GPU cycles to ns
+TW_CYCLES = (1.0, 10.0, 100.0, 1000.0)
 METRIC = "G probes/s on 1 MI355X + % random-access roofline; tuned vs plain config"
 
 
@@ -46,11 +59,13 @@ def distinct_keys(torch, n, seed, high_bit):
     return x.to(torch.int32)
 
 
-def make_inputs(torch, log2_m, n_probe, hit_rate, seed):
-    """Seeded synthetic inputs on the GPU: m/12 distinct build keys in
-    [0, 2^31), probe keys distinct in [2^31, 2^32) with round(hit_rate * n) of
-    them replaced (at random positions) by random build keys."""
-    n_build = (1 << log2_m) // 12
+def make_inputs(torch, log2_m, n_probe, hit_rate, seed, n_build=None):
+    """Seeded synthetic inputs on the GPU: n_build (default m/12) distinct
+    build keys in [0, 2^31), probe keys distinct in [2^31, 2^32) with
+    round(hit_rate * n) of them replaced (at random positions) by random build
+    keys."""
+    if n_build is None:
+        n_build = (1 << log2_m) // 12
     g = torch.Generator(device="cuda").manual_seed(seed)
     build = distinct_keys(torch, n_build, 2 * seed, 0)
     probe = distinct_keys(torch, n_probe, 2 * seed + 1, 1)
@@ -60,14 +75,15 @@ def make_inputs(torch, log2_m, n_probe, hit_rate, seed):
     return build, probe, n_build
 
 
-def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_step=None, warm_ms=0.0):
-    """Returns (ms per step, avg ms of the probe kernel, matches).  If
-    per_step is a dict it receives the median, min and max of the per-call
-    HIP-event times (ms).  Warm-up runs `warmup` calls and, with warm_ms,
-    keeps calling until that much wall time has passed (GPU clocks ramp up
-    over tens of milliseconds after idle)."""
+def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_step=None, warm_ms=0.0, out=None):
+    """Returns (ms per step, avg ms of the probe phase, matches, positions
+    tensor).  If per_step is a dict it receives the median, min and max of
+    the per-call HIP-event times (ms).  Warm-up runs `warmup` calls and, with
+    warm_ms, keeps calling until that much wall time has passed (GPU clocks
+    ramp up over tens of milliseconds after idle)."""
     n = probe.numel()
-    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    if out is None:
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     stream = torch.cuda.current_stream()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
@@ -83,6 +99,7 @@ def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_ste
         torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     t0.record(stream)
     for i in range(steps):
         p = D.ProbeParams() if params is None else D.ProbeParams(
@@ -101,7 +118,8 @@ def time_probe(torch, D, desc, words, probe, steps, warmup, params=None, per_ste
         mid = len(each) // 2
         per_step.update({"median": round(each[mid] if len(each) % 2 else 0.5 * (each[mid - 1] + each[mid]), 4),
                          "min": round(each[0], 4), "max": round(each[-1], 4), "calls": len(each)})
-    return total_ms / steps, kern_ms, int(cnt.item())
+    c = int(cnt.item())
+    return total_ms / steps, kern_ms, c, out[:c]
 
 
 def host_cpu_topology():
@@ -129,31 +147,47 @@ def host_cpu_topology():
             "smt": per_core > 1, "model": model}
 
 
-def cpu_baseline(torch, O, CB, words, probe, log2_m, sample_keys):
-    """The multithreaded AVX-512 restatement of the reference's batch_contains
-    (oracle/cpu_simd.cpp) on the first `sample_keys` probe keys against the
-    same 1 GiB filter copied to host memory: one thread per physical core of
-    this process's CPU share (at most 16), each probing a contiguous slice in
-    1024-key batches into its own output (benchmark.cpp:123-164); best of 3."""
+def cpu_best(run_once, min_seconds=0.25, min_reps=3, max_reps=50):
+    """Best wall time over repetitions that last at least min_seconds in all
+    (benchmark.cpp:166-193); returns (best seconds, last result, reps)."""
+    best, total, reps, last = None, 0.0, 0, None
+    while reps < min_reps or (total < min_seconds and reps < max_reps):
+        last = run_once()
+        best = last.seconds if best is None else min(best, last.seconds)
+        total += last.seconds
+        reps += 1
+    return best, last, reps
+
+
+def cpu_baseline(CB, kind, obj, data, keys_np, gpu_positions, what, one_thread=False):
+    """The oracle's threaded AVX-512 / AVX2 restatement of the reference's
+    batch_contains (oracle/cpu_simd.cpp) over the whole batch `keys_np`
+    against the same filter in host memory: one pinned worker per physical
+    core of this process's CPU share (at most 16), each probing a contiguous
+    slice in 1024-key batches into its own output (benchmark.cpp:123-164);
+    best of repetitions lasting >= 250 ms.  The concatenated list is compared
+    element by element with the GPU's (`gpu_positions`, uint64)."""
     import numpy as np
-    f = O.Bloom(64, 1 << log2_m, 8, 8, 8, 1)
-    host_words = words.cpu().numpy().view(np.uint64)
-    keys = probe[:sample_keys].cpu().numpy().view(np.uint32).copy()
     threads = min(CB.physical_cores(), 16)
-    simd = CB.bbf_simd_available(f)
-    best = None
-    for _ in range(3):
-        _, secs, _ = CB.bbf_probe(f, host_words, keys, threads, simd)
-        best = secs if best is None else min(best, secs)
-    one = keys[: max(sample_keys // 16, 1 << 20)]
-    _, secs1, _ = CB.bbf_probe(f, host_words, one, 1, simd)
-    return {"value": round(sample_keys / best / 1e9, 4), "unit": "G probes/s", "cores": threads,
-            "kind": "port", "host_cpu": host_cpu_topology(),
-            "one_thread_G_probes_per_s": round(len(one) / secs1 / 1e9, 4),
-            "sample": "%d probe keys (1%% hits) against the same 1 GiB filter in host memory; %s "
-                      "batch_contains, %d threads, best of 3 = %.3f s; 1 thread: %.4f G probes/s"
-                      % (sample_keys, "AVX-512" if simd else "scalar", threads, best,
-                         len(one) / secs1 / 1e9)}
+    n = len(keys_np)
+    rates = {}
+    last = None
+    for isa in ("avx512", "avx2"):
+        best, r, reps = cpu_best(lambda: CB.run(kind, obj, data, keys_np, threads, isa))
+        rates[r.isa] = (n / best / 1e9, best, reps)
+        if last is None:
+            last = r
+    identical = bool(np.array_equal(last.positions(), gpu_positions))
+    top = max(rates, key=lambda k: rates[k][0])
+    res = {"value": round(rates[top][0], 4), "unit": "G probes/s", "cores": threads, "kind": "port",
+           "isa": top, "by_isa_G_probes_per_s": {k: round(v[0], 4) for k, v in rates.items()},
+           "host_cpu": host_cpu_topology(), "lists_identical_to_gpu": identical,
+           "sample": "the whole batch: %d probe keys against %s in host memory; %s batch_contains, %d pinned "
+                     "threads, best of %d runs = %.4f s" % (n, what, top, threads, rates[top][2], rates[top][1])}
+    if one_thread:
+        r1 = CB.run(kind, obj, data, keys_np, 1, "avx512")
+        res["one_thread_G_probes_per_s"] = round(n / r1.seconds / 1e9, 4)
+    return res
 
 
 GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal",
@@ -188,41 +222,66 @@ def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
     return out
 
 
-def regime(torch, D, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, warmup, label):
-    """Whole-job probe rate of one filter configuration and size."""
+def regime(torch, D, CB, O, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, warmup, label,
+           n_build=None, no_cpu=False):
+    """Whole-job probe rate of one filter configuration and size, with the
+    CPU baseline on the same batch and their lists compared."""
+    import numpy as np
     desc = D.bbf_desc(word_bits, 1 << log2_m, k, w, s, z)
-    build, probe, n_build = make_inputs(torch, log2_m, n_probe, hit_rate, 7)
-    if word_bits == 32 and w == 1:  # the plain filter is built from 2^16 keys (its own config)
-        build = build[: 1 << 16]
+    build, probe, n_build = make_inputs(torch, log2_m, n_probe, hit_rate, 7, n_build=n_build)
     words = D.new_filter(desc)
     D.bbf_build(desc, words, build)
     torch.cuda.synchronize()
     st = {}
-    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st, warm_ms=200.0)
-    return {"config": label, "filter_bytes": int(desc.size_bytes),
-            "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
-            "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
-            "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
-            "call_ms": st, "matches": matches}
+    ms, kern_ms, matches, pos = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st, warm_ms=200.0)
+    bpk = 4.0 + 4.0 * matches / n_probe   # key + match from HBM; the filter is read once per workgroup or from cache
+    res = {"config": label, "filter_bytes": int(desc.size_bytes),
+           "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
+           "G_probes_per_s": round(n_probe / (ms * 1e-3) / 1e9, 2),
+           "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
+           "call_ms": st, "matches": matches,
+           "hbm_bytes_per_probe": round(bpk + (0.0 if desc.size_bytes <= 158 * 1024 else 64.0), 3)}
+    if not no_cpu:
+        f = O.Bloom(word_bits, 1 << log2_m, k, w, s, z)
+        gpu = pos.cpu().numpy().view(np.uint32).astype(np.uint64)
+        res["cpu_baseline"] = cpu_baseline(CB, "bbf", f, words.cpu().numpy().view(f.dtype),
+                                           probe.cpu().numpy().view(np.uint32), gpu, "the same filter")
+    del words, build, probe, pos
+    torch.cuda.empty_cache()
+    return res
 
 
-def run_regimes(torch, D, steps_arg):
-    """The filter-size sweep: 16 KiB (LDS), 4 MiB (L2), 128 MiB (Infinity
-    Cache) and the plain register-blocked 128 KiB filter, with the LDS
-    random-read rate beside the two LDS-staged ones."""
+def run_regimes(torch, D, CB, O, steps_arg, no_cpu=False):
+    """The filter-size sweep of the sectorized filter: 16 KiB (LDS), 4 MiB
+    (L2), 128 MiB (Infinity Cache), and the plain register-blocked 128 KiB
+    filter (2^16 build keys, 2^24 probes at 10% hits), with the LDS random
+    4-byte read rate beside the two LDS-staged ones."""
     log("bench: filter-size regimes")
     steps = max(3, steps_arg // 2)
     warm = 3
-    regs = [regime(torch, D, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 16 KiB"),
-            regime(torch, D, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 4 MiB"),
-            regime(torch, D, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 128 MiB"),
-            regime(torch, D, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, warm,
-                   "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits")]
+    regs = [regime(torch, D, CB, O, 64, 17, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 16 KiB",
+                   no_cpu=no_cpu),
+            regime(torch, D, CB, O, 64, 25, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 4 MiB",
+                   no_cpu=no_cpu),
+            regime(torch, D, CB, O, 64, 30, 8, 8, 8, 1, 1 << 28, 0.01, steps, warm, "u64 w8 s8 k8, 128 MiB",
+                   no_cpu=no_cpu),
+            regime(torch, D, CB, O, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, warm,
+                   "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits",
+                   n_build=1 << 16, no_cpu=no_cpu)]
     for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
         lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
-        r["lds_random_4B_reads"] = {"G_reads_per_s": lds["G_sectors_per_s"], "buffer_bytes": 1 << lb,
-                                    "probe_vs_lds_reads": round(r["kernel_G_probes_per_s"]
-                                                                / lds["G_sectors_per_s"], 3)}
+        r["roofline"] = {"bound": "lds", "achieved": round(r["kernel_G_probes_per_s"], 2),
+                         "peak": lds["G_sectors_per_s"], "unit": "G random 4-byte LDS reads/s",
+                         "frac": round(r["kernel_G_probes_per_s"] / lds["G_sectors_per_s"], 3),
+                         "note": "one LDS word read per probe at least; the LDS rate is measured "
+                                 "(dtl_gather_bench mode 8, buffer of %d bytes)" % (1 << lb)}
+    for r in regs[1:3]:
+        bpk = 4.0 + 64.0 + 4.0 * r["matches"] / (1 << 28)
+        ach = bpk * r["kernel_G_probes_per_s"]
+        r["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_probe": round(bpk, 3),
+                         "note": "4 B key + 64 B block + 4 B per match; the block is read from L2 / the "
+                                 "Infinity Cache, so this is an equivalent figure"}
     return regs
 
 
@@ -283,11 +342,9 @@ def build_filters(torch, D, meta, probes, choose):
     return tasks
 
 
-def time_graph_plan(torch, D, tasks, steps, tune_reps):
-    """Autotunes a batch plan over tasks, captures one run into a hipGraph and
-    times replays; returns (ms per replay, counts, tuned shape, groups)."""
-    plan = D.BatchPlan(tasks)
-    best = plan.tune(reps=tune_reps)
+def time_plan_graph(torch, plan, steps):
+    """Captures one run of a (tuned) batch plan into a hipGraph and times
+    replays; returns (ms per replay, per-task counts of the last replay)."""
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         plan.run(s)
@@ -306,19 +363,51 @@ def time_graph_plan(torch, D, tasks, steps, tune_reps):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / steps
     counts = [int(c.item()) for c in plan.counts]
-    groups = plan.group_count()
-    plan.close()
     del graph
-    return ms, counts, best, groups
+    return ms, counts
 
 
-def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), model=None, tune_reps=3):
+def semi_join_cpu_baseline(torch, CB, O, tasks, meta, gpu_lists, sample, seed=5):
+    """The CPU baseline of the semi-join on a seeded sample of `sample`
+    filters (their whole probe batches): pinned workers, one per physical
+    core, each probing whole filters (largest first) with the AVX-512 body of
+    the fixed configuration; best of repetitions lasting >= 250 ms.  Each
+    filter's list is compared with the GPU's."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    idx = sorted(rng.choice(len(tasks), size=min(sample, len(tasks)), replace=False).tolist(),
+                 key=lambda i: -meta[i]["n_probe"])
+    wb, k, w, s_, z = FIXED_SEMI_JOIN_CFG
+    blooms = [O.Bloom(wb, 10 * meta[i]["n_build"], k, w, s_, z) for i in idx]
+    words = [tasks[i][1].cpu().numpy().view(np.uint32) for i in idx]
+    keys = [tasks[i][2].cpu().numpy().view(np.uint32) for i in idx]
+    n = sum(len(x) for x in keys)
+    threads = min(CB.physical_cores(), 16)
+    best, reps, lists, total = None, 0, None, 0.0
+    while reps < 3 or (total < 0.25 and reps < 20):
+        lists, secs = CB.probe_tasks(blooms, words, keys, threads, "avx512")
+        best = secs if best is None else min(best, secs)
+        total += secs
+        reps += 1
+    identical = all(np.array_equal(l, gpu_lists[i]) for l, i in zip(lists, idx))
+    return {"value": round(n / best / 1e9, 4), "unit": "G probes/s", "cores": threads, "kind": "port",
+            "isa": "AVX-512" if CB.bbf_simd_available(blooms[0]) & 2 else "scalar",
+            "lists_identical_to_gpu": bool(identical),
+            "sample": "%d of the %d filters (seeded), %d probe keys: pinned workers probe whole filters, "
+                      "best of %d runs = %.4f s" % (len(idx), len(tasks), n, reps, best)}
+
+
+def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=None, tune_reps=3, CB=None, O=None,
+                cpu_sample=512):
     """The batched semi-join: plain per-filter launches of the fixed-config
     kernel vs one autotuned, graph-captured batch plan over all filters (same
     filters, same lists); with a trained cost model, also the learned form:
     each filter's (config, m) chosen by the model for its n and tw, all
-    filters in one mixed-configuration plan.  Total filter cost = probe time
-    + tw x false positives (measured: matches - true hits)."""
+    filters in one mixed-configuration plan, beside the exhaustive table's
+    choice in the same kind of plan.  Total filter cost = probe time + tw x
+    false positives (measured: matches - true hits); tw in GPU cycles (at
+    GPU_CLOCK_GHZ) and ns."""
+    import numpy as np
     log("batched: generating %d partitions" % n_filters)
     meta, probes = semi_join_inputs(torch, n_filters, seed)
     n_keys = sum(m["n_probe"] for m in meta)
@@ -344,11 +433,16 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
     plain_ms = a.elapsed_time(b) / steps
     plain_counts = [int(c.item()) for c in cnts]
     plain_matches = sum(plain_counts)
-    del outs, cnts
-    torch.cuda.empty_cache()
 
     log("batched: autotuning the batch plan")
-    batch_ms, counts, best, _ = time_graph_plan(torch, D, tasks, steps, tune_reps)
+    plan = D.BatchPlan(tasks)
+    best = plan.tune(reps=tune_reps)
+    batch_ms, counts = time_plan_graph(torch, plan, steps)
+    plan_lists = plan.results()
+    plain_lists = [D.positions(o, c) for o, c in zip(outs, cnts)]
+    lists_identical = all(np.array_equal(x, y) for x, y in zip(plan_lists, plain_lists))
+    plan.close()
+    del outs, cnts, plan_lists
     filter_bytes = int(sum(t[0].size_bytes for t in tasks))
     # Two byte counts per filter f with n keys and c matches.  Block model:
     # 4 B per key, one 64 B block per key where the filter is read from
@@ -366,10 +460,18 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
         tiles = (nk + 2047) // 2048
         block_bytes += 4 * nk + (desc.size_bytes * tiles if staged else 64 * nk) + 4 * c
         hbm_bytes += 4 * nk + min(desc.size_bytes, 64 * nk) + 4 * c
-    del tasks
+    cpu = None
+    if CB is not None:
+        log("batched: CPU baseline on %d filters" % cpu_sample)
+        cpu = semi_join_cpu_baseline(torch, CB, O, tasks, meta, plain_lists, cpu_sample)
+        cpu["full_workload_ms_at_this_rate"] = round(n_keys / cpu["value"] / 1e6, 1)
+    del tasks, plain_lists
     torch.cuda.empty_cache()
     fp = plain_matches - true_hits
-    res = {"filters": n_filters, "probe_keys": n_keys, "true_hits": true_hits,
+    res = {"workload": "%d zoned partition filters (u32 words, w=16, z=2, k=6, MAGIC, m = 10 bits per build "
+                       "key), build sizes log-uniform 2^10..2^22 keys, probe batches log-uniform 2^12..2^24 "
+                       "keys at hit rates uniform in [0.1%%, 30%%]" % n_filters,
+           "filters": n_filters, "probe_keys": n_keys, "true_hits": true_hits,
            "fixed_config": "u32 words, w=16, z=2, k=6, m=10 bits/key (MAGIC)",
            "filter_bytes": filter_bytes,
            "plain_ms": round(plain_ms, 3), "batched_graph_ms": round(batch_ms, 3),
@@ -378,6 +480,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
            "speedup": round(plain_ms / batch_ms, 3),
            "matches": plain_matches, "false_positives": fp,
            "counts_identical": bool(counts == plain_counts),
+           "lists_identical": bool(lists_identical),
            "tuned": {"lds_max_bytes": best.lds_max_bytes, "block_threads": best.block_threads,
                      "chunk_tiles": best.chunk_tiles},
            "roofline": {"bound": "hbm", "bytes_per_probe": round(hbm_bytes / n_keys, 2),
@@ -393,11 +496,15 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
                                                 "4 B per match; above 1 where block reads hit L2 or "
                                                 "the Infinity Cache"}},
            "total_filter_cost_ms": {}}
-    for t in tw_ns:
-        row = {"plain": round(plain_ms + t * fp * 1e-6, 3), "batched": round(batch_ms + t * fp * 1e-6, 3)}
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
+    for tc in tw_cycles:
+        t = tc / GPU_CLOCK_GHZ
+        row = {"tw_cycles": tc, "tw_ns": round(t, 4),
+               "plain": round(plain_ms + t * fp * 1e-6, 3), "batched": round(batch_ms + t * fp * 1e-6, 3)}
         if model is not None:
             for leg, use_mlp in (("learned_batched", True), ("exhaustive_batched", False)):
-                log("batched: %s configurations for tw=%g ns" % (leg, t))
+                log("batched: %s configurations for tw=%g cycles" % (leg, tc))
                 choices = {}
 
                 def chosen(f, nb):
@@ -407,7 +514,11 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
                     return ch.word_bits, ch.k, ch.w, ch.s, ch.z, ch.m
 
                 lt = build_filters(torch, D, meta, probes, chosen)
-                l_ms, l_counts, _, groups = time_graph_plan(torch, D, lt, steps, tune_reps)
+                lplan = D.BatchPlan(lt)
+                lplan.tune(reps=tune_reps)
+                l_ms, l_counts = time_plan_graph(torch, lplan, steps)
+                groups = lplan.group_count()
+                lplan.close()
                 l_fp = sum(l_counts) - true_hits
                 used = {}
                 for desc, _, _, _ in lt:
@@ -422,7 +533,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_ns=(1.0, 10.0, 100.0), m
             row["plain_over_learned_batched"] = round(row["plain"] / row["learned_batched"]["total_ms"], 3)
             row["learned_regret_vs_exhaustive"] = round(
                 row["learned_batched"]["total_ms"] / row["exhaustive_batched"]["total_ms"] - 1.0, 4)
-        res["total_filter_cost_ms"]["tw=%gns" % t] = row
+        res["total_filter_cost_ms"]["tw=%gcycles" % tc] = row
     del probes
     torch.cuda.empty_cache()
     return res
@@ -502,11 +613,12 @@ def cuckoo_probe_keys(torch, build_dev, n, hit):
     return probe, hits
 
 
-def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, load=0.95):
+def run_cuckoo(torch, D, CB, O, steps, warmup, no_cpu, log2_buckets=26, load=0.95):
     """Cuckoo filter, 16-bit tags, 4 slots per bucket, 2^26 buckets (512 MiB +
     victim cache), loaded to 95% with distinct uniform keys by the host build
     (cf_build_host, the reference's insert with a seeded mt19937 for kicks);
-    GPU probe of 2^28 keys at 1% and 50% hits."""
+    GPU probe of 2^28 keys at 1% and 50% hits; the GPU build (cf_build) of
+    the same keys timed beside it."""
     import numpy as np
     desc = D.cf_desc((1 << log2_buckets) * 64, 16, 4)
     n_build = int(load * (1 << log2_buckets) * 4)
@@ -519,7 +631,8 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
     dbuf = torch.from_numpy(buf).cuda()
     build_dev = torch.from_numpy(keys.view(np.int32)).cuda()
     del keys
-    out = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+    n = 1 << 28
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     res = {"workload": "cuckoo filter batch_contains: 16-bit tags, 4 slots/bucket, 2^%d buckets (%d MiB), "
                        "%d keys (%.0f%% load) built on the host; 2^28 probe keys" %
@@ -545,8 +658,9 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
                         "failed": int(failed.item()), "build_keys_found": int(found_cnt.item()),
                         "vs_host_build": round(build_s * 1e3 / g_ms, 1)}
     del gbuf, found_out
+    cf = O.Cuckoo(desc.m, 16, 4) if not no_cpu else None
     for hit in (0.01, 0.5):
-        n = 1 << 28
+        log("cuckoo: probing at %.0f%% hits" % (hit * 100))
         probe, hits = cuckoo_probe_keys(torch, build_dev, n, hit)
 
         def timed(params):
@@ -559,36 +673,30 @@ def run_cuckoo(torch, D, steps, warmup, cpu_sample, no_cpu, log2_buckets=26, loa
                 D.cf_probe(desc, dbuf, probe, 0, out=out, count=cnt, params=params)
             b.record()
             torch.cuda.synchronize()
-            return a.elapsed_time(b) / steps, int(cnt.item())
+            c = int(cnt.item())
+            return a.elapsed_time(b) / steps, c, out[:c].cpu().numpy().view(np.uint32).astype(np.uint64)
 
-        ms, matches = timed(None)   # default: partitioned two-pass probe for this size
-        d_ms, d_matches = timed(D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT))
+        ms, matches, pos = timed(None)   # default: partitioned two-pass probe for this size
+        d_ms, d_matches, d_pos = timed(D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT))
         bpk = 4.0 + 2 * 64.0 + 4.0 * matches / n
         leg = {"hit_rate": hit, "ms": round(ms, 4), "G_probes_per_s": round(n / ms / 1e6, 2),
                "matches": matches, "false_positives": matches - hits,
                "path": "partitioned (keys grouped by first-bucket slice, then by alternate-bucket slice)",
                "roofline": {"bound": "hbm", "bytes_per_probe": round(bpk, 2),
                             "achieved": round(bpk * n / ms / 1e6, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(bpk * n / ms / 1e6 / HBM_PEAK_GBS, 4)},
+                            "frac": round(bpk * n / ms / 1e6 / HBM_PEAK_GBS, 4),
+                            "note": "4 B key + two 64 B bucket sectors + 4 B per match"},
                "direct_kernel_call": {"ms": round(d_ms, 4), "G_probes_per_s": round(n / d_ms / 1e6, 2),
                                       "frac": round(bpk * n / d_ms / 1e6 / HBM_PEAK_GBS, 4),
-                                      "matches_identical": bool(d_matches == matches)}}
+                                      "lists_identical": bool(np.array_equal(d_pos, pos))}}
+        del d_pos
         if not no_cpu:
-            import cpu_baseline as CB
-            import oracle as O
-            cf = O.Cuckoo(desc.m, 16, 4)
-            hk = probe[:cpu_sample].cpu().numpy().view(np.uint32).copy()
-            threads = min(CB.physical_cores(), 16)
-            simd = CB.cf_simd_available(cf)
-            best = None
-            for _ in range(3):
-                _, secs, _ = CB.cf_probe(cf, buf, hk, threads, simd)
-                best = secs if best is None else min(best, secs)
-            leg["cpu_baseline"] = {"value": round(len(hk) / best / 1e9, 4), "unit": "G probes/s", "cores": threads,
-                                   "kind": "port", "sample": "%d probe keys, %s, best of 3" %
-                                   (len(hk), "AVX-512" if simd else "scalar")}
+            leg["cpu_baseline"] = cpu_baseline(CB, "cf", cf, buf, probe.cpu().numpy().view(np.uint32), pos,
+                                               "the same 512 MiB table")
         res["legs"].append(leg)
-        del probe
+        del probe, pos
+    del dbuf, build_dev, out
+    torch.cuda.empty_cache()
     return res
 
 
@@ -608,21 +716,22 @@ def main():
     ap.add_argument("--log2-m", type=int, default=33)
     ap.add_argument("--log2-probe", type=int, default=28)
     ap.add_argument("--hit-rate", type=float, default=0.01)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 28,
-                    help="probe keys of the CPU baseline sample (default: the whole batch, ~26 CPU-seconds)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip every CPU baseline")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline (no regimes, cuckoo, learned model or semi-join)")
+    ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
+    ap.add_argument("--no-cuckoo", action="store_true", help="skip the cuckoo workload")
+    ap.add_argument("--no-semi-join", action="store_true", help="skip the learned model and the semi-join")
+    ap.add_argument("--semi-join-filters", type=int, default=4096)
     ap.add_argument("--gather-only", action="store_true",
                     help="only run the random-sector gather microbenchmark (all access patterns)")
-    ap.add_argument("--no-regimes", action="store_true", help="skip the filter-size sweep")
-    ap.add_argument("--regimes-only", action="store_true",
-                    help="only run the filter-size sweep (16 KiB / 4 MiB / 128 MiB and the plain 128 KiB filter)")
+    ap.add_argument("--regimes-only", action="store_true", help="only run the filter-size sweep")
     ap.add_argument("--batched", type=int, default=0, metavar="FILTERS",
-                    help="only run the batched semi-join workload with this many filters (4096 = full)")
-    ap.add_argument("--cuckoo", action="store_true",
-                    help="only run the cuckoo filter workload (2^26 buckets at 95%% load, host build)")
+                    help="only run the batched semi-join workload with this many filters (fixed configuration)")
+    ap.add_argument("--cuckoo", action="store_true", help="only run the cuckoo filter workload")
     ap.add_argument("--learned", action="store_true",
-                    help="only run the learned configuration model (calibration, training, regret); with "
-                         "--batched also the learned + batched semi-join")
+                    help="only run the learned configuration model; with --batched also the learned + batched "
+                         "semi-join")
     ap.add_argument("--train-fraction", type=float, default=0.25)
     ap.add_argument("--model-path", default=None, help="save the calibrated + trained model here")
     ap.add_argument("--calib-copy", action="store_true",
@@ -633,33 +742,34 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     import dtl_hip as D
     D.load()
     torch.cuda.set_device(0)
+    CB = O = None
+    if not args.no_cpu:
+        import cpu_baseline as CB
+        import oracle as O
 
     if args.cuckoo:
-        print(json.dumps({"cuckoo": run_cuckoo(torch, D, args.steps, args.warmup, args.cpu_sample // 4,
-                                               args.no_cpu)}), flush=True)
+        print(json.dumps({"cuckoo": run_cuckoo(torch, D, CB, O, args.steps, args.warmup, args.no_cpu)}), flush=True)
         return
     if args.learned:
         res, model = run_learned(torch, D, train_fraction=args.train_fraction, model_path=args.model_path)
         print(json.dumps({"learned_config": res}), flush=True)
         if args.batched:
-            b = run_batched(torch, D, args.batched, max(2, args.steps // 4), tw_ns=(1.0, 10.0, 100.0, 1000.0),
-                            model=model, tune_reps=1)
+            b = run_batched(torch, D, args.batched, max(2, args.steps // 4), model=model, tune_reps=1, CB=CB, O=O)
             print(json.dumps({"batched_semi_join": b}), flush=True)
         model.close()
         return
     if args.batched:
-        print(json.dumps({"batched_semi_join": run_batched(torch, D, args.batched, max(2, args.steps // 4))}),
-              flush=True)
+        print(json.dumps({"batched_semi_join": run_batched(torch, D, args.batched, max(2, args.steps // 4),
+                                                           CB=CB, O=O)}), flush=True)
         return
-
     if args.regimes_only:
-        print(json.dumps({"regimes": run_regimes(torch, D, args.steps)}), flush=True)
+        print(json.dumps({"regimes": run_regimes(torch, D, CB, O, args.steps, args.no_cpu)}), flush=True)
         return
-
     if args.gather_only:
         res = {"random_sector_gather_1GiB": gather_table(torch, D, range(8))}
         for lb in (21, 22, 24, 27, 30):
@@ -684,17 +794,22 @@ def main():
 
     log("bench: probing (%d warmup + %d timed steps)" % (args.warmup, args.steps))
     per_call = {}
-    ms, kern_ms, matches = time_probe(torch, D, desc, words, probe, args.steps, args.warmup, per_step=per_call)
+    ms, kern_ms, matches, pos = time_probe(torch, D, desc, words, probe, args.steps, args.warmup, per_step=per_call)
     n = probe.numel()
     value = n / (ms * 1e-3) / 1e9
     partitioned = desc.size_bytes >= (64 << 20) and n >= (1 << 22)
-    bytes_per_key = 4.0 + 64.0 + 1.0 / 8.0   # key + one 64 B block + match bitmap bit
+    # algorithmic bytes per probe: 4 B key + one 64 B block + 4 B per match
+    bytes_per_key = 4.0 + 64.0 + 4.0 * matches / n
     achieved = bytes_per_key * n / (kern_ms * 1e-3) / 1e9
     traffic = load_traffic("traffic_bbf_probe_u64_w8_s8_k8_1gib.json")
+    gpu_pos = pos.cpu().numpy().view(np.uint32).astype(np.uint64)
+    del pos
     log("bench: direct kernel leg")
     dparams = D.ProbeParams(0, 0, 0, 0, D.PROBE_DIRECT)
-    d_ms, d_kern_ms, d_matches = time_probe(torch, D, desc, words, probe, max(3, args.steps // 2), 1,
-                                            params=dparams)
+    d_ms, d_kern_ms, d_matches, d_pos = time_probe(torch, D, desc, words, probe, max(3, args.steps // 2), 1,
+                                                   params=dparams)
+    d_identical = bool(np.array_equal(d_pos.cpu().numpy().view(np.uint32).astype(np.uint64), gpu_pos))
+    del d_pos
     d_achieved = bytes_per_key * n / (d_kern_ms * 1e-3) / 1e9
 
     result = {
@@ -725,27 +840,29 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "traffic_measured_at": traffic.get("commit") if traffic else None,
             "kernel": ("partitioned probe phase (k_part_hist, k_part_scatter, k_bbf_probe_split over the "
                        "grouped keys: 4 lanes per key, one 16 B piece of the 64 B block each; match "
                        "reorder), HIP events around the phase" if partitioned
                        else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
             "kernel_ms_per_call": per_call,
-            "bytes_per_probe": bytes_per_key,
+            "bytes_per_probe": round(bytes_per_key, 4),
             "direct_kernel": {
                 "kernel": "k_bbf_probe_global<u64,w8,s8,k8> (one random 64 B block per key from HBM)",
                 "kernel_ms": round(d_kern_ms, 4), "achieved": round(d_achieved, 1),
                 "frac": round(d_achieved / HBM_PEAK_GBS, 4),
                 "G_probes_per_s": round(n / (d_ms * 1e-3) / 1e9, 3),
-                "matches_identical": bool(d_matches == matches)},
+                "lists_identical": d_identical},
         },
     }
     if not args.no_cpu:
-        import cpu_baseline as CB
-        import oracle as O
-        log("bench: CPU baseline on %d keys" % args.cpu_sample)
-        result["cpu_baseline"] = cpu_baseline(torch, O, CB, words, probe, args.log2_m, args.cpu_sample)
-    del probe, words
+        log("bench: CPU baseline on the whole batch (%d keys)" % n)
+        f = O.Bloom(64, 1 << args.log2_m, 8, 8, 8, 1)
+        result["cpu_baseline"] = cpu_baseline(CB, "bbf", f, words.cpu().numpy().view(np.uint64),
+                                              probe.cpu().numpy().view(np.uint32), gpu_pos,
+                                              "the same 1 GiB filter", one_thread=True)
+    del probe, words, gpu_pos
     torch.cuda.empty_cache()
     g = gather_table(torch, D, [0, 2])
     best = max(g.values(), key=lambda r: r["G_sectors_per_s"])
@@ -754,8 +871,24 @@ def main():
         "direct_kernel_vs_gather": round(n / (d_kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
         "probe_phase_vs_gather": round(n / (kern_ms * 1e-3) / 1e9 / best["G_sectors_per_s"], 3),
         "note": "gfx950 fills a whole 128 B L2 line per random 64 B block (TCC_EA0_RDREQ_128B = all reads)"}
+    if args.headline_only:
+        print(json.dumps(result), flush=True)
+        return
     if not args.no_regimes:
-        result["regimes"] = run_regimes(torch, D, args.steps)
+        result["regimes"] = run_regimes(torch, D, CB, O, args.steps, args.no_cpu)
+    if not args.no_cuckoo:
+        result["cuckoo"] = run_cuckoo(torch, D, CB, O, max(3, args.steps // 2), 2, args.no_cpu)
+    if not args.no_semi_join:
+        lres, model = run_learned(torch, D, train_fraction=args.train_fraction, model_path=args.model_path)
+        result["learned_config"] = lres
+        result["batched_semi_join"] = run_batched(torch, D, args.semi_join_filters, max(2, args.steps // 4),
+                                                  model=model, tune_reps=1, CB=CB, O=O)
+        model.close()
+        result["tuned_vs_plain"] = {
+            k: {"plain_ms": v["plain"], "learned_batched_ms": v["learned_batched"]["total_ms"],
+                "speedup": v["plain_over_learned_batched"],
+                "learned_regret_vs_exhaustive": v["learned_regret_vs_exhaustive"]}
+            for k, v in result["batched_semi_join"]["total_filter_cost_ms"].items()}
     print(json.dumps(result), flush=True)
 
 

@@ -37,6 +37,10 @@ def lib():
             getattr(L, fn).argtypes = [C.POINTER(desc), C.c_void_p, C.c_void_p, C.c_uint64, C.c_int,
                                        C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.cpub_bbf_probe_tasks.restype = C.c_uint64
+        L.cpub_bbf_probe_tasks.argtypes = [C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_uint64), C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.cpub_bbf_build.restype = None
         L.cpub_bbf_build.argtypes = [C.POINTER(O.BBF), C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
         _lib = L
@@ -116,3 +120,20 @@ def bbf_build(bloom, keys, threads):
     words = bloom.new_words()
     lib().cpub_bbf_build(C.byref(bloom.d), words.ctypes.data, keys.ctypes.data, len(keys), threads)
     return words
+
+
+def probe_tasks(blooms, words, keys, threads, simd=True):
+    """Probes many filters, each with its own keys, on `threads` pinned
+    workers that take whole tasks in the given order (largest first balances
+    best).  Returns (per-task position lists, wall seconds)."""
+    n = len(blooms)
+    keys = [np.ascontiguousarray(k, dtype=np.uint32) for k in keys]
+    outs = [np.zeros(max(len(k), 1), dtype=np.uint32) for k in keys]
+    arr = lambda xs: (C.c_void_p * n)(*xs)
+    counts = (C.c_uint64 * n)()
+    secs = C.c_double(0)
+    lib().cpub_bbf_probe_tasks(n, arr([C.addressof(b.d) for b in blooms]), arr([w.ctypes.data for w in words]),
+                               arr([k.ctypes.data for k in keys]), (C.c_uint64 * n)(*[len(k) for k in keys]),
+                               arr([o.ctypes.data for o in outs]), counts, threads, _isa_code(simd),
+                               C.byref(secs))
+    return [o[: counts[i]] for i, o in enumerate(outs)], secs.value

@@ -29,6 +29,7 @@
 #include <sched.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -579,6 +580,39 @@ uint64_t cpub_cf_probe(const orc_cf* f, const void* buf, const uint32_t* keys, u
   });
   *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return c;
+}
+
+// Many filters, each with its own key batch (the semi-join): `threads`
+// pinned workers take whole tasks from a shared counter, in the given order,
+// and probe each with one thread in 1024-key batches (positions from 0).
+// Returns the total count; *seconds is the wall time of the parallel region.
+uint64_t cpub_bbf_probe_tasks(uint32_t n_tasks, const orc_bbf* const* filters, const void* const* words,
+                              const uint32_t* const* keys, const uint64_t* n, uint32_t* const* outs,
+                              uint64_t* counts, int threads, int isa, double* seconds) {
+  threads = std::max(1, threads);
+  static const std::vector<int> cpus = core_cpus();
+  std::atomic<uint32_t> next{0};
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    pool.emplace_back([&, t] {
+      if (g_pin && !cpus.empty()) {
+        cpu_set_t m;
+        CPU_ZERO(&m);
+        CPU_SET(cpus[t % cpus.size()], &m);
+        (void)pthread_setaffinity_np(pthread_self(), sizeof(m), &m);
+      }
+      for (uint32_t i = next.fetch_add(1); i < n_tasks; i = next.fetch_add(1)) {
+        const int use = pick_isa(isa, bbf_isas(filters[i]));
+        counts[i] = bbf_slice(filters[i], words[i], keys[i], n[i], outs[i], 0, use);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n_tasks; ++i) total += counts[i];
+  return total;
 }
 
 // Builds the filter from keys[0..n) with `threads` threads (a checker for

@@ -128,3 +128,31 @@ def test_sparse_and_block_checkers_match_dense_oracle(cfg):
     w = f.d.w
     gathered = dense.reshape(-1, w)[blocks]
     assert np.array_equal(f.batch_contains_blocks(gathered, probe, 9), f.batch_contains(dense, probe, 9))
+
+
+@pytest.mark.parametrize("isa", ISAS)
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_probe_tasks_matches_oracle_per_filter(isa, threads):
+    """The semi-join baseline (pinned workers taking whole filters) returns
+    each filter's own list, positions from 0, equal to the scalar oracle's."""
+    rng = np.random.default_rng(threads)
+    blooms, words, keys, exp = [], [], [], []
+    for i, n_build in enumerate([1 << 10, 3001, 1 << 14, 70001, 60, 1 << 17]):
+        f = O.Bloom(32, 10 * n_build, 6, 16, 16, 2)
+        if i == 0:
+            _needs(isa, CB.bbf_simd_available(f))
+        build = rng.integers(0, 2**31, n_build, dtype=np.uint64).astype(np.uint32)
+        w = f.new_words()
+        f.insert(w, build)
+        n_probe = int(rng.integers(1, 40000))
+        probe = rng.integers(0, 2**32, n_probe, dtype=np.uint64).astype(np.uint32)
+        probe[::5] = build[rng.integers(0, n_build, len(probe[::5]))]
+        blooms.append(f)
+        words.append(w.view(np.uint32))
+        keys.append(probe)
+        exp.append(f.batch_contains(w, probe))
+    lists, secs = CB.probe_tasks(blooms, words, keys, threads, isa)
+    assert secs > 0
+    assert len(lists) == len(exp)
+    for got, e in zip(lists, exp):
+        assert np.array_equal(got, e)
