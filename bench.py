@@ -20,7 +20,8 @@ timed region):
   batched_semi_join  4096 zoned partition filters: plain per-filter calls of the
                      fixed configuration vs one tuned, graph-captured batch plan,
                      and the learned vs exhaustive configurations in batch plans,
-                     total filter cost at tw = 1, 10, 100 and 1000 ns per tuple.
+                     total filter cost at tw = 1, 10, 100 and 1000 GPU cycles
+                     (at 2.4 GHz) per qualifying tuple.
 """
 import argparse
 import json
@@ -539,12 +540,12 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     return res
 
 
-def run_learned(torch, D, train_fraction=0.25, samples=4000000, epochs=16, model_path=None):
+def run_learned(torch, D, train_fraction=0.25, samples=2000000, epochs=32, model_path=None):
     """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
     bits, timed on the GPU; FPR measured per configuration and bits per key),
     MLP trained on a seeded `train_fraction` of the (configuration, size)
     cells, then the learned choice against the exhaustive optimum over
-    n = 2^10..2^28 and tw = 1..1000 ns per qualifying tuple."""
+    n = 2^10..2^28 and tw = 1..1000 GPU cycles per qualifying tuple."""
     import numpy as np
     log("learned: calibration sweep")
     M = D.CostModel.calibrate()
@@ -555,9 +556,10 @@ def run_learned(torch, D, train_fraction=0.25, samples=4000000, epochs=16, model
     train_s = time.time() - t0
     regrets, rows = [], []
     t_ml = t_ex = 0.0
-    tws = (1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 200.0, 500.0, 1000.0)
+    tws = (1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 200.0, 500.0, 1000.0)   # GPU cycles per qualifying tuple
     for ln in range(10, 29, 2):
-        for tw in tws:
+        for tc in tws:
+            tw = tc / GPU_CLOCK_GHZ
             t0 = time.time()
             ml = M.choose(1 << ln, tw, use_mlp=True)
             t1 = time.time()
@@ -567,16 +569,18 @@ def run_learned(torch, D, train_fraction=0.25, samples=4000000, epochs=16, model
             t_ex += t2 - t1
             r = ml.cost_ns / ex.cost_ns - 1.0
             regrets.append(r)
-            if tw in (1.0, 100.0) and ln in (10, 20, 28):
-                rows.append({"n": 1 << ln, "tw_ns": tw, "learned": ml.as_dict(), "exhaustive": ex.as_dict(),
-                             "regret": round(r, 4)})
+            if tc in (1.0, 100.0) and ln in (10, 20, 28):
+                rows.append({"n": 1 << ln, "tw_cycles": tc, "tw_ns": round(tw, 4), "learned": ml.as_dict(),
+                             "exhaustive": ex.as_dict(), "regret": round(r, 4)})
     regrets = np.array(regrets)
     if model_path:
         M.save(model_path)
     res = {"configurations": M.config_count(), "filter_sizes_log2": [13, 33],
            "calibration_s": round(calib_s, 2), "train_s": round(train_s, 2),
            "train_fraction_of_cells": train_fraction, "train_samples": samples,
-           "queries": int(regrets.size), "regret_mean": round(float(regrets.mean()), 4),
+           "mlp": "8-64-64-1 FP32 (word size, w, s, z, k, log2 m, log2 tw, log2 calibrated FPR at m/n bits per key)",
+           "queries": int(regrets.size), "query_grid": "n = 2^10..2^28 (x4) x tw = 1..1000 cycles (10 values)",
+           "regret_mean": round(float(regrets.mean()), 4), "regret_median": round(float(np.median(regrets)), 4),
            "regret_p95": round(float(np.percentile(regrets, 95)), 4),
            "regret_max": round(float(regrets.max()), 4),
            "share_within_5pct": round(float((regrets <= 0.05).mean()), 3),

@@ -1,10 +1,10 @@
-// dtl_mlp.hip -- the 10-64-64-1 FP32 multilayer perceptron of the learned
+// dtl_mlp.hip -- the 8-64-64-1 FP32 multilayer perceptron of the learned
 // configuration model: hand-written training (forward, backward, Adam) and
 // inference kernels.  No reference counterpart: it stands in for the
 // reference's exhaustive calibration + skyline lookup
 // (model/calibration.cpp:251-311, tool/calibration/src/skyline_matrix_builder.cpp).
 //
-// Parameter layout (floats): W1[64][10] b1[64] W2[64][64] b2[64] W3[64] b3.
+// Parameter layout (floats): W1[64][8] b1[64] W2[64][64] b2[64] W3[64] b3.
 // Training: each workgroup takes 64 samples of the minibatch, keeps the
 // activations in LDS, forms its share of the gradient with small LDS matrix
 // products (dW2 = dH2^T H1 etc.) and adds it to the global gradient with
@@ -21,7 +21,7 @@
 
 namespace dtlh {
 
-constexpr int kIn = 10, kH = 64;
+constexpr int kIn = 8, kH = 64;
 constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
               oB3 = oW3 + kH, kParams = oB3 + 1;
 constexpr int kRows = 64;  // samples per training workgroup
@@ -60,11 +60,17 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ p
 
 // Gradient of 0.5 * mean(w * (f(x) - t)^2) over the minibatch rows
 // idx(j) = (a * (base + j) + c) mod n, j < count (an epoch-specific
-// permutation); w = 1 when no weights are given.
+// permutation); w = wts[idx] (1 when no weights are given), plus, when
+// `best` is given, dyn * exp(-max(f(x) - best[idx], 0) * inv_temp): rows the
+// network currently predicts close to (or below) the best cost of their
+// query weigh more, so a poor candidate predicted to be good is corrected.
+// That term is a weight, held constant in the gradient.
 __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ params,
                                                   const float* __restrict__ x,
                                                   const float* __restrict__ t,
-                                                  const float* __restrict__ wts, uint32_t n,
+                                                  const float* __restrict__ wts,
+                                                  const float* __restrict__ best, float dyn,
+                                                  float inv_temp, uint32_t n,
                                                   uint64_t perm_a, uint64_t perm_c, uint32_t base,
                                                   uint32_t count, float* __restrict__ grad) {
   __shared__ float p[kParams];
@@ -111,7 +117,9 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ para
       float yv = p[oB3];
       for (int k = 0; k < kH; ++k) yv += p[oW3 + k] * H2[tid][k];
       const uint64_t idx = (perm_a * (base + row0 + tid) + perm_c) % n;
-      dv = (yv - t[idx]) * (wts ? wts[idx] : 1.0f) / static_cast<float>(count);
+      float wv = wts ? wts[idx] : 1.0f;
+      if (best) wv += dyn * __expf(-fmaxf(yv - best[idx], 0.f) * inv_temp);
+      dv = (yv - t[idx]) * wv / static_cast<float>(count);
     }
     dy[tid] = dv;
   }
@@ -265,13 +273,21 @@ int dtl_mlp_set_params(dtl_mlp* m, const float* h_params) {
              : DTL_EHIP;
 }
 
-// Trains on n samples (d_x: n x kIn = 10 features, d_t: n targets, d_w: n sample
+// Trains on n samples (d_x: n x kIn = 8 features, d_t: n targets, d_w: n sample
 // weights or null; device memory) with Adam, minibatches of `batch` rows,
 // `epochs` passes; each epoch visits the rows in the order (a * j + c) mod n
 // for a fresh (a coprime to n, c).
 int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream) {
-  if (!m || !d_x || !d_t || n == 0 || batch == 0) return DTL_EINVAL;
+  return dtl_mlp_train_ex(m, d_x, d_t, d_w, nullptr, 0.f, 1.f, n, epochs, batch, lr, seed, stream);
+}
+
+// As dtl_mlp_train, with the prediction-dependent weight of k_mlp_grad when
+// d_best is given (temp > 0, in the units of the targets).
+int dtl_mlp_train_ex(dtl_mlp* m, const float* d_x, const float* d_t, const float* d_w, const float* d_best,
+                     float dyn, float temp, uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                     dtl_stream_t stream) {
+  if (!m || !d_x || !d_t || n == 0 || batch == 0 || !(temp > 0.f)) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   (void)hipGetLastError();
   float* params = m->d;
@@ -301,8 +317,8 @@ int dtl_mlp_train(dtl_mlp* m, const float* d_x, const float* d_t, const float* d
     for (uint32_t b0 = 0; b0 < n; b0 += batch) {
       const uint32_t cnt = std::min(batch, n - b0);
       const uint32_t blocks = (cnt + kRows - 1) / kRows;
-      hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, st, params, d_x, d_t, d_w, n, a, c, b0,
-                         cnt, grad);
+      hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, st, params, d_x, d_t, d_w, d_best, dyn,
+                         1.f / temp, n, a, c, b0, cnt, grad);
       m->step += 1;
       const float b1t = std::pow(0.9f, static_cast<float>(m->step));
       const float b2t = std::pow(0.999f, static_cast<float>(m->step));
@@ -327,8 +343,8 @@ int dtl_mlp_gradient(const dtl_mlp* m, const float* d_x, const float* d_t, const
   hipStream_t st = static_cast<hipStream_t>(stream);
   (void)hipGetLastError();
   hipLaunchKernelGGL(k_fill_zero, dim3((kParams + 255) / 256), dim3(256), 0, st, d_grad, kParams);
-  hipLaunchKernelGGL(k_mlp_grad, dim3((n + kRows - 1) / kRows), dim3(256), 0, st, m->d, d_x, d_t, d_w, n,
-                     uint64_t(1), uint64_t(0), 0u, n, d_grad);
+  hipLaunchKernelGGL(k_mlp_grad, dim3((n + kRows - 1) / kRows), dim3(256), 0, st, m->d, d_x, d_t, d_w,
+                     static_cast<const float*>(nullptr), 0.f, 1.f, n, uint64_t(1), uint64_t(0), 0u, n, d_grad);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     set_last_error(err);

@@ -5,10 +5,13 @@
 // Objective (the reference's, cost_fn.hpp:143-166):
 //   overhead(config, m, n, tw) = t_lookup(config, m) + tw * fpr(config, m, n)
 // per probe, in nanoseconds here (tw = downstream work per qualifying tuple).
-//   * t_lookup: measured GPU probe time per key (whole bbf_probe call, 2^24
-//     keys, zero filter) at filter sizes 2^13 .. 2^33 bits, interpolated in
-//     log2(m) -- the GPU analogue of the reference's per-memory-level timings
-//     (model/calibration.cpp:251-311, benchmark.cpp:104-199);
+//   * t_lookup: measured GPU probe time per key in the execution the chosen
+//     filters run in -- a batch plan (the persistent batched kernel, many
+//     filters with their own key batches) -- at filter sizes 2^13 .. 2^33
+//     bits, interpolated in log2(m): the GPU analogue of the reference's
+//     per-memory-level timings under its own multithreaded batch execution
+//     (cost_fn.hpp:77-137, model/calibration.cpp:251-311,
+//     benchmark.cpp:104-199);
 //   * fpr: measured on the GPU for every configuration at 8 bits-per-key
 //     points (n = 2^16 built keys, 2^24 absent keys probed), interpolated in
 //     log(fpr) over bits per key.
@@ -105,43 +108,25 @@ double table_cost(const dtl_cost_model& M, int c, double m, double n, double tw)
   return interp_t(M, c, std::log2(m)) + tw * interp_fpr(M, c, m / n);
 }
 
-// Inputs of the MLP: the configuration (word size, w, s, z, k, block bytes
-// and the lanes per key of its lane-split probe kernel, which changes its
-// lookup cost in steps), filter size, bits per key and tw.
-constexpr int kFeat = 10;
+// Inputs of the MLP (8): the configuration (word size, w, s, z, k), log2 of
+// the filter size m, log2 tw, and log2 of the configuration's calibrated
+// false-positive rate at m / n bits per key -- the input through which n
+// enters: the objective depends on n only through fpr(config, m / n), as in
+// the reference's cost function (cost_fn.hpp:158-166), and the FPR sweep is
+// the cheap part of the calibration (every configuration is measured), while
+// the MLP stands in for the expensive per-(configuration, size) timing sweep.
+constexpr int kFeat = 8;
 
-uint32_t ilog2(uint32_t v) { return 31u - static_cast<uint32_t>(__builtin_clz(v)); }
-
-// Lanes per key of the lane-split kernel, looked up once per configuration
-// (thread-safe static initialisation; features() runs on several threads).
-uint32_t split_lanes_of(const Cfg& c) {
-  auto key = [](uint32_t wb, uint32_t w, uint32_t s, uint32_t z, uint32_t k) {
-    return ((wb == 64 ? 1u : 0u) << 13) | (ilog2(w) << 10) | (ilog2(s) << 7) | (ilog2(z) << 5) | (k & 31u);
-  };
-  static const std::vector<uint8_t> table = [&] {
-    std::vector<uint8_t> t(1u << 14, 0);
-    for (uint32_t wb : {32u, 64u})
-      for (uint32_t w = 1; w <= 16; w *= 2)
-        for (uint32_t s = 1; s <= 16; s *= 2)
-          for (uint32_t z = 1; z <= 8; z *= 2)
-            for (uint32_t k = 1; k <= 16; ++k)
-              t[key(wb, w, s, z, k)] = static_cast<uint8_t>(dtl_bbf_split_lanes(wb, w, s, z, k));
-    return t;
-  }();
-  return table[key(c.word_bits, c.w, c.s, c.z, c.k)];
-}
-
-void features(const Cfg& c, double m, double n, double tw, float* f) {
+void features(const dtl_cost_model& M, int ci, double m, double n, double tw, float* f) {
+  const Cfg& c = M.cfgs[ci];
   f[0] = c.word_bits / 64.0f;
   f[1] = static_cast<float>(std::log2(c.w) / 4.0);
   f[2] = static_cast<float>(std::log2(c.s) / 4.0);
   f[3] = static_cast<float>(std::log2(c.z) / 2.0);
   f[4] = c.k / 16.0f;
   f[5] = static_cast<float>((std::log2(m) - 23.0) / 10.0);
-  f[6] = static_cast<float>(std::log2(m / n) - 3.3);  // bits per key
-  f[7] = static_cast<float>(std::log2(tw) / 10.0);
-  f[8] = static_cast<float>(std::log2(c.word_bits / 8.0 * c.w) / 7.0);  // block bytes
-  f[9] = split_lanes_of(c) / 8.0f;
+  f[6] = static_cast<float>(std::log2(tw) / 10.0);
+  f[7] = static_cast<float>(std::log2(interp_fpr(M, ci, m / n)) / 10.0);
 }
 
 struct DevBuf {
@@ -171,6 +156,210 @@ std::vector<double> candidate_m(const dtl_cost_model& M, const Cfg& c, uint64_t 
   return out;
 }
 
+constexpr uint32_t kCalTasks = 32;
+constexpr uint32_t kCalPlanLds = 32 * 1024, kCalPlanThreads = 256, kCalPlanChunk = 4;
+constexpr double kCalTargetMs = 0.5;     // aimed duration of a timed run
+constexpr uint64_t kCalEstKeys = 1ull << 17;  // keys per task of the estimation run (at most)
+
+// Fills M.t_ns (ns per probe key) for every (config, size) from batch-plan
+// runs: kCalTasks tasks, each probing its own zero filter of the size (tasks
+// share one filter from 64 MiB on) with K absent keys (windows of `keys` that
+// start at evenly spaced offsets), run by the persistent batched kernel at the
+// plan's standard launch shape (LDS staging up to 32 KiB, 256-thread
+// workgroups, 4 tiles per grab) with the per-task compaction.
+// Per size, all configurations are timed in one enqueued sequence (no host
+// round trip between runs, so the GPU stays busy and its clock steady), in
+// `passes` passes that alternate the order; a configuration's cost per key is
+//   (median of its runs with K keys per task - median of its runs with one
+//    tile per task) / (kCalTasks * (K - one tile)),
+// which removes the fixed cost of the launches.  K is the power of two that
+// makes a run last about kCalTargetMs, from an estimation pass.
+int calibrate_plans(dtl_cost_model& M, const uint32_t* keys, uint64_t nkeys, uint32_t passes, hipStream_t st) {
+  int dev = 0, cus = 0;
+  uint32_t max_lds = 0;
+  int rc = current_device(&dev, &cus, &max_lds);
+  if (rc != DTL_OK) return rc;
+  const uint64_t kmax = nkeys / 4;  // keys per task at most
+  if (kmax < 2ull * kTileKeys || (kmax & (kmax - 1))) return DTL_EINVAL;
+  const uint64_t kest = std::min(kCalEstKeys, kmax);
+  const int nc = static_cast<int>(M.cfgs.size()), ns = M.nsizes();
+  const uint64_t max_fbytes = (1ull << M.log2_max) / 8;
+  const uint64_t share_from = 64ull << 20;   // tasks share one filter from this size on
+  const uint64_t stride = std::min(max_fbytes, share_from);
+  // the largest filter can exceed max_fbytes by a few blocks (MAGIC sizes round up)
+  const uint64_t filt_bytes = std::max(stride * kCalTasks, max_fbytes) + (1u << 20);
+  const uint64_t tiles_max = kCalTasks * (kmax / kTileKeys);
+  // tile -> task maps, one per K (powers of two from one tile to kmax)
+  std::vector<uint32_t> maps;
+  std::vector<uint64_t> map_at;   // indexed by log2(K / kTileKeys)
+  for (uint64_t K = kTileKeys; K <= kmax; K *= 2) {
+    map_at.push_back(maps.size());
+    const uint64_t per = K / kTileKeys;
+    for (uint32_t t = 0; t < kCalTasks; ++t) maps.insert(maps.end(), per, t);
+  }
+  // two task tables per configuration: one tile per task (slot 0), K keys per task (slot 1)
+  const size_t ntab = static_cast<size_t>(nc) * 2;
+  DevBuf filt, pos, cnt, tasks, tt, bits, tcnt;
+  // positions: one buffer for every task -- a zero filter matches nothing, and
+  // a task writes at most n <= kmax positions, so any write stays in bounds
+  if (!filt.alloc(filt_bytes) || !pos.alloc(kmax * 4) || !cnt.alloc(kCalTasks * 8) ||
+      !tasks.alloc(sizeof(BatchTaskDev) * kCalTasks * ntab) || !tt.alloc(maps.size() * 4) ||
+      !bits.alloc(tiles_max * kTileWords * 8) || !tcnt.alloc((tiles_max + 1) * 4))
+    return DTL_ENOMEM;
+  if (hipMemsetAsync(filt.p, 0, filt_bytes, st) != hipSuccess ||
+      hipMemcpyAsync(tt.p, maps.data(), maps.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return DTL_EHIP;
+  const size_t nev = ntab + 1;
+  std::vector<hipEvent_t> ev(nev, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      rc = DTL_EHIP;
+      break;
+    }
+  std::vector<BatchTaskDev> host(kCalTasks * ntab);
+  std::vector<const BbfEntry*> entry(nc);
+  for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+    entry[c] = bbf_find(M.cfgs[c].word_bits, M.cfgs[c].w, M.cfgs[c].s, M.cfgs[c].z, M.cfgs[c].k);
+    if (!entry[c] || !entry[c]->probe_batched) rc = DTL_EINVAL;
+  }
+  BbfBatchLaunch L{};
+  L.n_tasks = kCalTasks;
+  L.bits = static_cast<uint64_t*>(bits.p);
+  L.tile_cnt = static_cast<uint32_t*>(tcnt.p);
+  L.work_counter = static_cast<uint32_t*>(tcnt.p) + tiles_max;
+  L.lds_max_bytes = std::min(kCalPlanLds, max_lds > 2048 ? max_lds - 2048 : 0u);
+  L.threads = kCalPlanThreads;
+  L.grid = 0;
+  L.chunk = kCalPlanChunk;
+  L.cu_count = cus;
+  L.stream = st;
+  std::vector<dtl_bbf_desc> desc(nc);
+  std::vector<uint64_t> kbig(nc);
+  // task table of configuration c, slot (0 or 1), with K keys per task
+  auto fill = [&](int c, int slot, uint64_t K) {
+    const dtl_bbf_desc& d = desc[c];
+    const uint64_t fstride = (d.size_bytes + 255) / 256 * 256;
+    const bool share = d.size_bytes > share_from || fstride * kCalTasks > filt_bytes;
+    BatchTaskDev* h = &host[(static_cast<size_t>(c) * 2 + slot) * kCalTasks];
+    const uint64_t step = (nkeys - K) / (kCalTasks - 1) & ~63ull;
+    for (uint32_t t = 0; t < kCalTasks; ++t) {
+      h[t].words = static_cast<char*>(filt.p) + (share ? 0 : t * fstride);
+      h[t].keys = keys + t * step;   // t * step + K <= nkeys
+      h[t].pos = static_cast<uint32_t*>(pos.p);
+      h[t].count = static_cast<uint64_t*>(cnt.p) + t;
+      h[t].n = static_cast<uint32_t>(K);
+      h[t].tile_begin = static_cast<uint32_t>(t * (K / kTileKeys));
+      h[t].tile_count = static_cast<uint32_t>(K / kTileKeys);
+      h[t].match_offset = 0;
+      if (d.addr_mode == DTL_ADDR_POW2) {
+        h[t].addr_magic = h[t].addr_mshift = h[t].addr_divisor = 0;
+        h[t].addr_post = d.block_shift;
+      } else {
+        h[t].addr_magic = d.magic;
+        h[t].addr_mshift = d.magic_shift;
+        h[t].addr_divisor = d.block_cnt;
+        h[t].addr_post = 0;
+      }
+      h[t].filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d.size_bytes, 0xFFFFFFFFull));
+    }
+  };
+  auto upload = [&]() -> int {
+    if (hipMemcpyAsync(tasks.p, host.data(), host.size() * sizeof(BatchTaskDev), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
+      return DTL_EHIP;
+    return DTL_OK;
+  };
+  // one run of configuration c from table `slot` with K keys per task
+  auto enqueue = [&](int c, int slot, uint64_t K) -> hipError_t {
+    uint32_t lvl = 0;
+    while ((kTileKeys << lvl) < K) ++lvl;
+    L.tasks = static_cast<const BatchTaskDev*>(tasks.p) + (static_cast<size_t>(c) * 2 + slot) * kCalTasks;
+    L.tile_task = static_cast<const uint32_t*>(tt.p) + map_at[lvl];
+    L.total_tiles = static_cast<uint32_t>(kCalTasks * (K / kTileKeys));
+    hipError_t he = zero_u32(L.work_counter, st);
+    if (he == hipSuccess) he = entry[c]->probe_batched(L);
+    if (he == hipSuccess) he = compact_tasks(L.bits, L.tile_cnt, L.tasks, L.tile_task, kCalTasks, L.total_tiles, st);
+    return he;
+  };
+  // one pass: every configuration's slot-0 run then slot-1 run, events between
+  // (order reversed when `reverse`); out[2c + slot] = ms
+  auto pass = [&](bool reverse, const std::vector<uint64_t>& K1, std::vector<float>& out) -> int {
+    hipError_t he = hipEventRecord(ev[0], st);
+    for (int i = 0; i < nc && he == hipSuccess; ++i) {
+      const int c = reverse ? nc - 1 - i : i;
+      for (int slot = 0; slot < 2 && he == hipSuccess; ++slot) {
+        he = enqueue(c, slot, slot ? K1[c] : kTileKeys);
+        if (he == hipSuccess) he = hipEventRecord(ev[2 * i + slot + 1], st);
+      }
+    }
+    if (he == hipSuccess) he = hipEventSynchronize(ev[ntab]);
+    for (int i = 0; i < nc && he == hipSuccess; ++i) {
+      const int c = reverse ? nc - 1 - i : i;
+      for (int slot = 0; slot < 2 && he == hipSuccess; ++slot)
+        he = hipEventElapsedTime(&out[2 * c + slot], ev[2 * i + slot], ev[2 * i + slot + 1]);
+    }
+    if (he != hipSuccess) {
+      set_last_error(he);
+      return DTL_EHIP;
+    }
+    return DTL_OK;
+  };
+  const uint32_t np = std::max(1u, passes);
+  std::vector<float> run(ntab);
+  std::vector<std::vector<float>> small(nc), big(nc);
+  for (int si = 0; si < ns && rc == DTL_OK; ++si) {
+    for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+      const Cfg& C = M.cfgs[c];
+      // a non-power-of-two size (MAGIC addressing), as chosen sizes m = b*n are
+      const uint64_t m = (1ull << (M.log2_min + si)) - 2ull * C.word_bits * C.w;
+      rc = dtl_bbf_desc_init(&desc[c], C.word_bits, std::max<uint64_t>(m, 3ull * C.word_bits * C.w), C.k, C.w, C.s,
+                             C.z);
+      if (rc != DTL_OK) break;
+      fill(c, 0, kTileKeys);
+      fill(c, 1, kest);
+      kbig[c] = kest;
+    }
+    // estimation pass: K for each configuration
+    if (rc == DTL_OK) rc = upload();
+    if (rc == DTL_OK) rc = pass(false, kbig, run);
+    for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+      const double per_key_ms = std::max(1e-12, static_cast<double>(run[2 * c + 1] - run[2 * c])) /
+                                (static_cast<double>(kCalTasks) * (kest - kTileKeys));
+      uint64_t K = kTileKeys * 2;
+      while (K < kmax && per_key_ms * kCalTasks * K * 1.41 < kCalTargetMs) K *= 2;
+      kbig[c] = K;
+      fill(c, 1, K);
+      small[c].clear();
+      big[c].clear();
+    }
+    if (rc == DTL_OK) rc = upload();
+    for (uint32_t p = 0; p < np && rc == DTL_OK; ++p) {
+      rc = pass(p & 1u, kbig, run);
+      for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+        small[c].push_back(run[2 * c]);
+        big[c].push_back(run[2 * c + 1]);
+      }
+    }
+    for (int c = 0; c < nc && rc == DTL_OK; ++c) {
+      auto median = [](std::vector<float>& v) {
+        std::sort(v.begin(), v.end());
+        const size_t h = v.size() / 2;
+        return v.size() % 2 ? static_cast<double>(v[h]) : 0.5 * (static_cast<double>(v[h - 1]) + v[h]);
+      };
+      const double t1 = median(big[c]), t0 = median(small[c]);
+      const double keys_run = static_cast<double>(kCalTasks) * static_cast<double>(kbig[c]);
+      const double d = t1 - t0;
+      // a difference lost in noise (never seen at these run lengths) falls back to the mean
+      M.t_ns[static_cast<size_t>(c) * ns + si] =
+          (d > 0.25 * t1 ? d / (keys_run - static_cast<double>(kCalTasks) * kTileKeys) : t1 / keys_run) * 1e6;
+    }
+  }
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -180,10 +369,10 @@ int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_str
   *out = nullptr;
   const uint32_t lmin = (p && p->log2_m_min) ? p->log2_m_min : 13;
   const uint32_t lmax = (p && p->log2_m_max) ? p->log2_m_max : 33;
-  const uint32_t lkeys = (p && p->probe_keys_log2) ? p->probe_keys_log2 : 24;
+  const uint32_t lkeys = (p && p->probe_keys_log2) ? p->probe_keys_log2 : 27;
   const uint32_t reps = (p && p->reps) ? p->reps : 3;
   const uint32_t lfpr = (p && p->fpr_keys_log2) ? p->fpr_keys_log2 : 24;
-  if (lmin < 10 || lmax > 34 || lmin > lmax || lkeys > 30) return DTL_EINVAL;
+  if (lmin < 10 || lmax > 34 || lmin > lmax || lkeys < 14 || lkeys > 30) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const auto t0 = std::chrono::steady_clock::now();
   dtl_cost_model* M = new dtl_cost_model();
@@ -225,34 +414,14 @@ int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_str
       if (rc == DTL_OK && hipStreamSynchronize(st) != hipSuccess) rc = DTL_EHIP;
     }
   }
-  // lookup cost per (config, size): a zero filter, so every key is a
-  // negative probe (the common case) and the time covers the whole call
-  for (int c = 0; c < nc && rc == DTL_OK; ++c) {
-    const Cfg& C = M->cfgs[c];
-    for (int si = 0; si < ns && rc == DTL_OK; ++si) {
-      // a non-power-of-two size (MAGIC addressing), as chosen sizes m = b*n are
-      const uint64_t m = (1ull << (lmin + si)) - 2ull * C.word_bits * C.w;
-      dtl_bbf_desc d;
-      rc = dtl_bbf_desc_init(&d, C.word_bits, std::max<uint64_t>(m, 3ull * C.word_bits * C.w), C.k, C.w,
-                             C.s, C.z);
-      if (rc != DTL_OK) break;
-      std::vector<float> ts;
-      for (uint32_t r = 0; r <= reps && rc == DTL_OK; ++r) {
-        if (hipEventRecord(a, st) != hipSuccess) rc = DTL_EHIP;
-        if (rc == DTL_OK)
-          rc = bbf_probe(&d, filt.p, static_cast<uint32_t*>(keys.p), nkeys, 0, static_cast<uint32_t*>(pos.p),
-                         static_cast<uint64_t*>(cnt.p), stream);
-        if (rc == DTL_OK && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess))
-          rc = DTL_EHIP;
-        float ms = 0;
-        if (rc == DTL_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = DTL_EHIP;
-        if (r > 0) ts.push_back(ms);
-      }
-      if (rc != DTL_OK) break;
-      std::sort(ts.begin(), ts.end());
-      M->t_ns[static_cast<size_t>(c) * ns + si] = ts[ts.size() / 2] * 1e6 / static_cast<double>(nkeys);
-    }
-  }
+  // lookup cost per (config, size), in a batch plan: kCalTasks tasks, each
+  // probing its own zero filter of that size (every key a negative probe, the
+  // common case; tasks share one filter from 64 MiB on), run by the
+  // persistent batched kernel at the plan's standard launch shape (LDS
+  // staging up to 32 KiB, 256-thread workgroups, 4 tiles per grab) with the
+  // per-task compaction.  The cost per key is the slope between runs of K and
+  // 2K keys per task, so the fixed cost of the launches drops out.
+  if (rc == DTL_OK) rc = calibrate_plans(*M, static_cast<uint32_t*>(keys.p), nkeys, reps, st);
   // false-positive rate per (config, bits per key), measured on real filters
   if (rc == DTL_OK && gen_keys(static_cast<uint32_t*>(absent.p), nfpr, 4242, 1, st) != hipSuccess) rc = DTL_EHIP;
   for (int c = 0; c < nc && rc == DTL_OK; ++c) {
@@ -320,20 +489,31 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 // Trains the MLP on a seeded `train_fraction` of the calibrated (config,
 // size) cells (the rest of the table is never seen by the MLP).  Rows: a random cell,
 // bits per key log-uniform in [4, 20] (the candidate range) and tw
-// log-uniform in [1, 1000] ns, target y = log(overhead) standardised.  The
-// loss weights a row by exp(-(y - y_best) / 0.1) + 0.2, y_best = the lowest
-// y of any training cell at the same (size, bits per key, tw): the choice
+// log-uniform in [kTwMinNs, kTwMaxNs], target y = log(overhead) standardised.  The
+// loss weights a row by exp(-(y - y_best) / kTemp) + kFloor, y_best = the
+// lowest y of any training cell at the same (size, bits per key, tw), plus a
+// term for rows the network currently predicts near y_best (kDyn): the choice
 // only depends on the ranking near the optimum, so that is where the fit has
-// to be tight.  kRestarts networks (different initialisations) are trained
+// to be tight, and a poor candidate predicted to be good is what makes a wrong
+// choice.  The MLP (two hidden layers of 64) cannot fit the whole table to a
+// few percent, so it spends its capacity there.  kRestarts networks (different initialisations) are trained
 // and the one whose choices have the lowest mean regret on a query grid,
 // measured on the training cells only (candidates at calibrated sizes, table
 // cost of those cells), is kept.
 constexpr int kRestarts = 4;
+// Row weights: exp(-(y - y_best) / kTemp) + kFloor from the table, plus
+// kDyn * exp(-max(prediction - y_best, 0) / kTemp) during training (rows the
+// network currently ranks near the optimum; dtl_mlp_train_ex).
+constexpr double kTemp = 0.1, kFloor = 0.05;
+constexpr float kDyn = 1.0f;
+// Range of tw the MLP is trained on, ns per qualifying tuple: one GPU cycle
+// at 2.4 GHz (0.42 ns) and below, up to 1000 ns.
+constexpr double kTwMinNs = 0.25, kTwMaxNs = 1000.0;
 
 namespace {
 
 struct Rows {
-  std::vector<float> X, Y, W;
+  std::vector<float> X, Y, W, B;  // features, log overhead, weight, log of the best overhead of the row's query
 };
 
 // Draws `count` rows from `cells`; y_best over every cell in `ref`.
@@ -350,6 +530,7 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
   r.X.assign(static_cast<size_t>(count) * kFeat, 0.f);
   r.Y.assign(count, 0.f);
   r.W.assign(count, 0.f);
+  r.B.assign(count, 0.f);
   std::vector<int> s_si(count);
   std::vector<double> s_bpk(count), s_tw(count);
   for (uint32_t i = 0; i < count; ++i) {
@@ -357,8 +538,8 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
     const int c = cell / ns, si = cell % ns;
     const double m = std::ldexp(1.0, M.log2_min + si);
     const double bpk = 4.0 * std::pow(5.0, unif());
-    const double tw = std::pow(1000.0, unif());
-    features(M.cfgs[c], m, m / bpk, tw, &r.X[static_cast<size_t>(i) * kFeat]);
+    const double tw = kTwMinNs * std::pow(kTwMaxNs / kTwMinNs, unif());
+    features(M, c, m, m / bpk, tw, &r.X[static_cast<size_t>(i) * kFeat]);
     r.Y[i] = static_cast<float>(std::log(M.t_ns[cell] + tw * interp_fpr(M, c, bpk)));
     s_si[i] = si;
     s_bpk[i] = bpk;
@@ -374,7 +555,8 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
           const size_t cell = static_cast<size_t>(c) * ns + s_si[i];
           if (ref[cell]) best = std::min(best, M.t_ns[cell] + s_tw[i] * interp_fpr(M, c, s_bpk[i]));
         }
-        r.W[i] = static_cast<float>(std::exp(-(r.Y[i] - std::log(best)) / 0.1) + 0.2);
+        r.B[i] = static_cast<float>(std::log(best));
+        r.W[i] = static_cast<float>(std::exp(-(r.Y[i] - r.B[i]) / kTemp) + kFloor);
       }
     });
   for (auto& x : th) x.join();
@@ -415,7 +597,8 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
   M->y_mean = sum / samples;
   M->y_std = std::sqrt(std::max(1e-12, sum2 / samples - M->y_mean * M->y_mean));
   for (auto& v : fit.Y) v = static_cast<float>((v - M->y_mean) / M->y_std);
-  // selection queries: n = 2^10 .. 2^28, tw = 1 .. 1000 ns; candidates are
+  for (auto& v : fit.B) v = static_cast<float>((v - M->y_mean) / M->y_std);
+  // selection queries: n = 2^10 .. 2^28, tw = 0.42 .. 1000 ns; candidates are
   // the training cells whose size gives 4..20 bits per key
   struct Query {
     uint32_t begin, end;
@@ -424,7 +607,7 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
   std::vector<float> qx;
   std::vector<double> qcost;
   for (int ln = 10; ln <= 28; ln += 2)
-    for (double tw : {1.0, 3.0, 10.0, 30.0, 100.0, 300.0, 1000.0}) {
+    for (double tw : {0.42, 1.0, 3.0, 10.0, 30.0, 100.0, 300.0, 1000.0}) {
       Query q{static_cast<uint32_t>(qcost.size()), 0};
       const double n = std::ldexp(1.0, ln);
       for (int c = 0; c < nc; ++c)
@@ -433,27 +616,31 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
           const double m = std::ldexp(1.0, M->log2_min + si), bpk = m / n;
           if (bpk < 4.0 || bpk > 20.0) continue;
           float f[kFeat];
-          features(M->cfgs[c], m, n, tw, f);
-          qx.insert(qx.end(), f, f + 8);
+          features(*M, c, m, n, tw, f);
+          qx.insert(qx.end(), f, f + kFeat);
           qcost.push_back(M->t_ns[static_cast<size_t>(c) * ns + si] + tw * interp_fpr(*M, c, bpk));
         }
       q.end = static_cast<uint32_t>(qcost.size());
       if (q.end > q.begin + 1) queries.push_back(q);
     }
   const uint32_t nq = static_cast<uint32_t>(std::max<size_t>(qcost.size(), 1));
-  DevBuf dx, dy, dw, dvx, dvy;
+  DevBuf dx, dy, dw, db, dvx, dvy;
   if (!dx.alloc(fit.X.size() * 4) || !dy.alloc(fit.Y.size() * 4) || !dw.alloc(fit.W.size() * 4) ||
-      !dvx.alloc(static_cast<size_t>(nq) * 32) || !dvy.alloc(static_cast<size_t>(nq) * 4))
+      !db.alloc(fit.B.size() * 4) ||
+      !dvx.alloc(static_cast<size_t>(nq) * kFeat * 4) || !dvy.alloc(static_cast<size_t>(nq) * 4))
     return DTL_ENOMEM;
   if (hipMemcpyAsync(dx.p, fit.X.data(), fit.X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dy.p, fit.Y.data(), fit.Y.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dw.p, fit.W.data(), fit.W.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(db.p, fit.B.data(), fit.B.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
       (!qx.empty() && hipMemcpyAsync(dvx.p, qx.data(), qx.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess))
     return DTL_EHIP;
   const float* px = static_cast<float*>(dx.p);
   const float* py = static_cast<float*>(dy.p);
   const float* pw = static_cast<float*>(dw.p);
-  const uint32_t ep = epochs ? epochs : 16;
+  const float* pb = static_cast<float*>(db.p);
+  const float tz = static_cast<float>(kTemp / M->y_std);  // kTemp in standardised units
+  const uint32_t ep = epochs ? epochs : 32;
   if (M->mlp) dtl_mlp_destroy(M->mlp);
   M->mlp = nullptr;
   std::vector<float> best_params(dtl_mlp_param_count()), pred(nq);
@@ -465,9 +652,11 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     rc = dtl_mlp_create(sd, &net);
     if (rc != DTL_OK) break;
     // learning-rate schedule: 3 phases
-    rc = dtl_mlp_train(net, px, py, pw, samples, ep / 2 + 1, 512, 2e-3f, sd, stream);
-    if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 4 + 1, 1024, 5e-4f, sd + 1, stream);
-    if (rc == DTL_OK) rc = dtl_mlp_train(net, px, py, pw, samples, ep / 8 + 1, 2048, 1e-4f, sd + 2, stream);
+    rc = dtl_mlp_train_ex(net, px, py, pw, pb, kDyn, tz, samples, ep / 2 + 1, 512, 2e-3f, sd, stream);
+    if (rc == DTL_OK)
+      rc = dtl_mlp_train_ex(net, px, py, pw, pb, kDyn, tz, samples, ep / 4 + 1, 1024, 5e-4f, sd + 1, stream);
+    if (rc == DTL_OK)
+      rc = dtl_mlp_train_ex(net, px, py, pw, pb, kDyn, tz, samples, ep / 8 + 1, 2048, 1e-4f, sd + 2, stream);
     const uint32_t npred = static_cast<uint32_t>(qcost.size());
     if (rc == DTL_OK && npred)
       rc = dtl_mlp_predict(net, static_cast<float*>(dvx.p), npred, static_cast<float*>(dvy.p), stream);
@@ -520,7 +709,7 @@ int dtl_model_choose(const dtl_cost_model* M, uint64_t n, double tw_ns, int use_
   if (use_mlp) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::vector<float> X(nk * kFeat), Y(nk);
-    for (size_t i = 0; i < nk; ++i) features(M->cfgs[cand_c[i]], cand_m[i], static_cast<double>(n), tw_ns, &X[i * kFeat]);
+    for (size_t i = 0; i < nk; ++i) features(*M, cand_c[i], cand_m[i], static_cast<double>(n), tw_ns, &X[i * kFeat]);
     DevBuf dx, dy;
     if (!dx.alloc(X.size() * 4) || !dy.alloc(nk * 4)) return DTL_ENOMEM;
     if (hipMemcpyAsync(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) return DTL_EHIP;
@@ -559,8 +748,8 @@ double dtl_model_cost(const dtl_cost_model* M, const dtl_bbf_choice* ch, uint64_
 }
 
 // Persistence: a small binary file (magic, version, grid, tables, MLP weights).
-// Version 2: 10 MLP inputs.
-constexpr uint32_t kModelVersion = 2;
+// Version 3: 8 MLP inputs, the last one the calibrated FPR.
+constexpr uint32_t kModelVersion = 3;
 int dtl_model_save(const dtl_cost_model* M, const char* path) {
   if (!M || !path) return DTL_EINVAL;
   FILE* f = std::fopen(path, "wb");

@@ -84,7 +84,7 @@ EXPORTS = [
     "bbf_batch_plan_group_count", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
     "dtl_model_fpr", "dtl_model_config_count", "dtl_model_get_config", "dtl_model_calibration_seconds",
     "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
-    "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
+    "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_train_ex", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
     "dtl_mlp_set_params", "dtl_mlp_input_count", "dtl_scratch_release", "dtl_scratch_bytes",
     "bbf_probe_noalloc", "dtl_noalloc_max_keys",
 ]
@@ -160,6 +160,7 @@ def load(path=LIB_PATH):
     L.dtl_mlp_destroy.argtypes = [vp]
     L.dtl_mlp_destroy.restype = None
     L.dtl_mlp_train.argtypes = [vp, vp, vp, vp, u32, u32, u32, f32, u32, vp]
+    L.dtl_mlp_train_ex.argtypes = [vp, vp, vp, vp, vp, f32, f32, u32, u32, u32, f32, u32, vp]
     L.dtl_mlp_predict.argtypes = [vp, vp, u32, vp, vp]
     L.dtl_mlp_gradient.argtypes = [vp, vp, vp, vp, u32, vp, vp]
     L.dtl_mlp_get_params.argtypes = [vp, C.POINTER(C.c_float)]
@@ -167,7 +168,7 @@ def load(path=LIB_PATH):
     for name in ("dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_config_count",
                  "dtl_model_get_config", "dtl_model_save", "dtl_model_load", "dtl_mlp_param_count",
                  "dtl_mlp_input_count", "dtl_scratch_release",
-                 "dtl_mlp_create", "dtl_mlp_train", "dtl_mlp_predict", "dtl_mlp_gradient",
+                 "dtl_mlp_create", "dtl_mlp_train", "dtl_mlp_train_ex", "dtl_mlp_predict", "dtl_mlp_gradient",
                  "dtl_mlp_get_params", "dtl_mlp_set_params"):
         getattr(L, name).restype = i32
     for name in ("dtl_bbf_desc_init", "dtl_cf_desc_init", "bbf_build", "bbf_probe", "bbf_probe_ex", "bbf_probe_noalloc",
@@ -419,7 +420,7 @@ def gather_bench(buf, mode, n_reads, sink, stream=None):
 
 
 class MLP:
-    """The library's FP32 MLP (dtl_mlp.hip): input_count (10) inputs, two
+    """The library's FP32 MLP (dtl_mlp.hip): input_count (8) inputs, two
     hidden layers of 64, one output."""
 
     def __init__(self, seed=0):
@@ -517,7 +518,7 @@ class CostModel:
     def save(self, path):
         _check("dtl_model_save", self.L.dtl_model_save(self.h, path.encode()))
 
-    def train(self, train_fraction=0.25, samples=4000000, epochs=16, seed=1):
+    def train(self, train_fraction=0.25, samples=2000000, epochs=32, seed=1):
         _check("dtl_model_train", self.L.dtl_model_train(self.h, train_fraction, samples, epochs, seed,
                                                          _stream(None)))
 

@@ -267,13 +267,14 @@ int bbf_probe_batched(const dtl_bbf_task* tasks, uint32_t n_tasks, const dtl_pro
  * lookup that picks (word size, w, s, z, k, m) from (n, tw)
  * (model/calibration.cpp:251-311, skyline_matrix.hpp:118-155,
  * optimizer.hpp:25-140) with GPU-timed calibration, measured FPRs and an FP32
- * 10-64-64-1 MLP; the calibrated table is the fallback and the yardstick of
+ * 8-64-64-1 MLP; the calibrated table is the fallback and the yardstick of
  * the MLP's regret. */
 typedef struct dtl_calib_params {
   uint32_t log2_m_min;      /* smallest calibrated filter, log2 bits (default 13) */
   uint32_t log2_m_max;      /* largest, log2 bits (default 33 = 1 GiB) */
-  uint32_t probe_keys_log2; /* keys per timing run (default 24) */
-  uint32_t reps;            /* timed repetitions, median kept (default 3) */
+  uint32_t probe_keys_log2; /* log2 of the absent keys the timing runs draw from, 14..30 (default 27);
+                               a run's tasks take windows of up to a quarter of them */
+  uint32_t reps;            /* timed passes over every configuration, median kept (default 3) */
   uint32_t fpr_keys_log2;   /* absent keys per FPR measurement (default 24) */
   uint32_t flags;
 } dtl_calib_params;
@@ -302,7 +303,7 @@ int dtl_model_save(const dtl_cost_model* model, const char* path);
 int dtl_model_load(const char* path, dtl_cost_model** model);
 void dtl_model_destroy(dtl_cost_model* model);
 
-/* The MLP itself: FP32, dtl_mlp_input_count() inputs (10), two hidden layers
+/* The MLP itself: FP32, dtl_mlp_input_count() inputs (8), two hidden layers
  * of 64 (ReLU), one output; parameters W1[64][IN] b1[64] W2[64][64] b2[64]
  * W3[64] b3 with IN = dtl_mlp_input_count(). */
 typedef struct dtl_mlp dtl_mlp;
@@ -311,11 +312,18 @@ int dtl_mlp_param_count(void);
 int dtl_mlp_create(uint32_t seed, dtl_mlp** mlp);
 void dtl_mlp_destroy(dtl_mlp* mlp);
 /* Adam on 0.5 * mean(w * (f(x) - t)^2); d_x: n rows of dtl_mlp_input_count()
- * floats (row-major, n x 10), d_t: n targets, d_w: n sample weights or NULL
+ * floats (row-major, n x 8), d_t: n targets, d_w: n sample weights or NULL
  * (all 1); device memory.  dtl_mlp_predict and dtl_mlp_gradient read d_x the
  * same way; a shorter buffer is read past its end. */
 int dtl_mlp_train(dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed, dtl_stream_t stream);
+/* As dtl_mlp_train; with d_best (n floats, device) each row's weight gains
+ * dyn * exp(-max(f(x) - d_best[i], 0) / temp), held constant in the gradient:
+ * rows predicted near or below the best target of their query weigh more
+ * (temp > 0, in target units). */
+int dtl_mlp_train_ex(dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, const float* d_best,
+                     float dyn, float temp, uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                     dtl_stream_t stream);
 int dtl_mlp_predict(const dtl_mlp* mlp, const float* d_x, uint32_t n, float* d_y, dtl_stream_t stream);
 int dtl_mlp_gradient(const dtl_mlp* mlp, const float* d_x, const float* d_t, const float* d_w, uint32_t n,
                      float* d_grad, dtl_stream_t stream);

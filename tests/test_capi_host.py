@@ -279,25 +279,25 @@ def test_python_binding_rejects_undersized_and_mismatched_buffers():
 
 
 def test_mlp_input_width_is_queried_and_documented():
-    """The MLP reads rows of dtl_mlp_input_count() = 10 floats; the header
+    """The MLP reads rows of dtl_mlp_input_count() = 8 floats; the header
     says so, the parameter count follows from it, and the binding refuses
     feature matrices of any other width before a launch."""
     import torch
     L = D.load()
     n_in = L.dtl_mlp_input_count()
-    assert n_in == 10
+    assert n_in == 8
     assert L.dtl_mlp_param_count() == 64 * n_in + 64 + 64 * 64 + 64 + 64 + 1
     header = open(os.path.join(ROOT, "include", "dtl_hip.h")).read()
-    assert "n x 10" in header and "W1[64][IN]" in header
-    assert "n x 8" not in header and "W1[64][8]" not in header
+    assert "n x 8" in header and "W1[64][IN]" in header
+    assert "n x 10" not in header and "W1[64][10]" not in header
     mlp = D.MLP.__new__(D.MLP)      # the argument checks only; creating one needs a GPU
     mlp.L, mlp.h, mlp.n_inputs, mlp.n_params = L, None, n_in, L.dtl_mlp_param_count()
     F = _FakeDeviceTensor
-    for bad in (F(64 * 8, 4, torch.float32, shape=(64, 8)), F(64 * 11, 4, torch.float32, shape=(64, 11)),
-                F(640, 4, torch.float32, shape=(640,)), F(640, 8, torch.float64, shape=(64, 10))):
+    for bad in (F(64 * 10, 4, torch.float32, shape=(64, 10)), F(64 * 7, 4, torch.float32, shape=(64, 7)),
+                F(512, 4, torch.float32, shape=(512,)), F(512, 8, torch.float64, shape=(64, 8))):
         with pytest.raises(ValueError, match="x"):
             mlp.predict(bad)
-    x = F(640, 4, torch.float32, shape=(64, 10))
+    x = F(512, 4, torch.float32, shape=(64, 8))
     with pytest.raises(ValueError, match="t"):
         mlp.gradient(x, F(63, 4, torch.float32))
     with pytest.raises(ValueError, match="w"):
