@@ -449,7 +449,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   const size_t bits_bytes = static_cast<size_t>(ntiles) * kTileWords * 8;
   const size_t cnt_bytes = (static_cast<size_t>(ntiles) + kScanTmpWords) * 4;
   const uint32_t flags = params ? params->flags : 0u;
-  uint32_t lds_max = di.max_lds - 1024;  // room for the kernel's own shared arrays
+  uint32_t lds_max = di.max_lds - 2048;  // room for the kernel's own shared arrays
   if (params && params->lds_max_bytes) lds_max = std::min(lds_max, params->lds_max_bytes);
   const bool use_lds = !(flags & DTL_PROBE_FORCE_GLOBAL) && d->size_bytes <= lds_max;
   // Lane split pays where blocks come from L2 / Infinity Cache (1.33x at
@@ -543,6 +543,8 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));
   }
 
+  uint32_t threads = (params && params->block_threads) ? params->block_threads : 0;
+  if (threads == 0) threads = d->size_bytes > 40 * 1024 ? 1024 : 256;
   if (!(flags & (DTL_PROBE_LANE_SPLIT | DTL_PROBE_ONE_LANE)) && d->size_bytes >= (256ull << 20)) split = 0;
   void* scratch = nullptr;
   rc = get_scratch(dev, st, bits_bytes + cnt_bytes, &scratch);
@@ -557,8 +559,6 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
   L.ntiles = ntiles;
   L.filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d->size_bytes, 0xFFFFFFFFull));
   L.use_lds = use_lds;
-  uint32_t threads = (params && params->block_threads) ? params->block_threads : 0;
-  if (threads == 0) threads = d->size_bytes > 40 * 1024 ? 1024 : 256;
   L.lds_threads = threads >= 1024 ? 1024 : 256;
   L.lds_grid = params ? params->grid_blocks : 0;
   L.split = split;

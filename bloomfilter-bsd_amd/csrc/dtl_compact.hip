@@ -10,6 +10,8 @@
 // (blocked_bloomfilter_batch_dispatch.hpp:47-67).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dtl_internal.hpp"
 #include "dtl_kernels.hpp"
 #include "dtl_scan.hpp"

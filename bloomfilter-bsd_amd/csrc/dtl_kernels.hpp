@@ -46,13 +46,26 @@ __device__ __forceinline__ uint32_t block_of(uint32_t h0, const AddrArgs& a) {
   return (h0 - q * a.divisor) >> a.post;
 }
 
-__device__ __forceinline__ uint32_t hslot(uint32_t key, int src) {
-  return src == 0 ? 0u : key * kMul.v[src];
-}
+// The hash values a key's chunks are cut from, each computed once:
+// h[i] = key * kMul[i] for 1 <= i <= NSRC, h[0] = 0.  One product per hash
+// function matters on gfx950, where a 32-bit multiply issues at a quarter of
+// the rate of other VALU operations: written as one expression per chunk, the
+// compiler narrows each product's constant to the bits that chunk needs
+// (different constants) and can no longer share the products.
+template <int NSRC>
+struct KeyHashes {
+  uint32_t h[NSRC + 1];
+  __device__ __forceinline__ explicit KeyHashes(uint32_t key) {
+    h[0] = 0;
+    static_for<NSRC>([&](auto i) { h[decltype(i)::value + 1] = key * kMul.v[decltype(i)::value + 1]; });
+  }
+  __device__ __forceinline__ uint32_t chunk(Chunk c) const {
+    return ((h[c.src] >> c.shift) & ((1u << c.nbits) - 1u)) + c.base;
+  }
+};
 
-__device__ __forceinline__ uint32_t chunk_val(uint32_t key, Chunk c) {
-  return ((hslot(key, c.src) >> c.shift) & ((1u << c.nbits) - 1u)) + c.base;
-}
+template <class C>
+using HashesOf = KeyHashes<C::sched.max_src>;
 
 template <class W>
 __device__ __forceinline__ W one_at(uint32_t bit) {
@@ -61,13 +74,13 @@ __device__ __forceinline__ W one_at(uint32_t bit) {
 
 // Per-group search masks of a key.
 template <class C>
-__device__ __forceinline__ void key_masks(uint32_t key, typename C::word_t (&mask)[C::sched.ngroups]) {
+__device__ __forceinline__ void key_masks(const HashesOf<C>& H, typename C::word_t (&mask)[C::sched.ngroups]) {
   using W = typename C::word_t;
   static_for<C::sched.ngroups>([&](auto g) { mask[g] = 0; });
   static_for<C::sched.nbits>([&](auto b) {
     constexpr Chunk c = C::sched.bbit[b];
     constexpr int g = C::sched.bgroup[b];
-    mask[g] |= one_at<W>(chunk_val(key, c));
+    mask[g] |= one_at<W>(H.chunk(c));
   });
 }
 
@@ -105,8 +118,9 @@ __device__ __forceinline__ void load_block(const typename C::word_t* __restrict_
 template <class C>
 __device__ __forceinline__ bool test_regs(uint32_t key, const typename C::word_t (&blk)[C::w]) {
   using W = typename C::word_t;
+  const HashesOf<C> H(key);
   W mask[C::sched.ngroups];
-  key_masks<C>(key, mask);
+  key_masks<C>(H, mask);
   bool ok = true;
   static_for<C::sched.ngroups>([&](auto g) {
     constexpr Chunk d = C::sched.gdyn[g];
@@ -115,7 +129,7 @@ __device__ __forceinline__ bool test_regs(uint32_t key, const typename C::word_t
     if constexpr (d.nbits == 0) {
       word = blk[base];
     } else {
-      word = select_word<base, (1 << d.nbits)>(blk, chunk_val(key, d));
+      word = select_word<base, (1 << d.nbits)>(blk, H.chunk(d));
     }
     ok &= (word & mask[g]) == mask[g];
   });
@@ -126,14 +140,15 @@ __device__ __forceinline__ bool test_regs(uint32_t key, const typename C::word_t
 template <class C, class P>
 __device__ __forceinline__ bool test_ptr(uint32_t key, P blk) {
   using W = typename C::word_t;
+  const HashesOf<C> H(key);
   W mask[C::sched.ngroups];
-  key_masks<C>(key, mask);
+  key_masks<C>(H, mask);
   bool ok = true;
   static_for<C::sched.ngroups>([&](auto g) {
     constexpr Chunk d = C::sched.gdyn[g];
     constexpr int base = C::sched.gword[g];
     uint32_t wi = base;
-    if constexpr (d.nbits != 0) wi += chunk_val(key, d);
+    if constexpr (d.nbits != 0) wi += H.chunk(d);
     const W word = blk[wi];
     ok &= (word & mask[g]) == mask[g];
   });
@@ -529,17 +544,32 @@ struct LdsLayout {
 
 // ---------------------------------------------------------------------------
 // Blocked Bloom filter probe, filter staged in LDS (persistent grid).
-// Workgroup b probes tiles b, b + grid, b + 2 grid, ...; it requests the keys
-// of G of its tiles at once (G x ROUNDS = 16 keys per lane: 8 tiles with the
-// 1024-thread block that large filters leave one of per CU, 2 with 256
-// threads), then tests them, so that key loads overlap even at one
-// workgroup per CU.  A tile past the end is
-// probed like the others (its keys are clamped, its hits masked) and only its
-// stores are skipped: the same loop with a uniform early exit per tile lost
-// hits of the group's first tile now and then on gfx950 (ROCm 7.2), which
-// this branch-free form does not.
+// Workgroup b probes tiles b, b + grid, b + 2 grid, ... in steps of G tiles
+// (G x ROUNDS = 16 keys per lane: 8 tiles with the 1024-thread block that
+// large filters leave one of per CU, 2 with 256 threads).  The keys of the
+// next step are requested before the current step is tested, so their
+// latency overlaps the test even at one workgroup per CU.  Lane j of a wave
+// keeps the ballot word of the step's j-th round, and the wave stores its 16
+// words with one 16-lane store at the end of the step.  A tile past the end
+// is probed like the others (its keys are clamped, its hits masked) and only
+// its stores are skipped: the loop stays branch-free.  POW2 addressing
+// (block = h >> post) and MAGIC (block = h mod d) are separate
+// instantiations.
 // ---------------------------------------------------------------------------
-template <class C, int BT>
+template <int BT, int G, int ROUNDS>
+__device__ __forceinline__ void lds_load_step(const uint32_t* __restrict__ keys, uint32_t n, uint32_t t0,
+                                              uint32_t grid, uint32_t tid, uint32_t (&key)[G][ROUNDS]) {
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * grid;
+      const uint64_t i = t * kTileKeys + static_cast<uint64_t>(r) * BT + tid;
+      key[g][r] = load_key(keys, i < n ? i : n - 1);  // n >= 1: clamp, no branch
+    }
+}
+
+template <class C, int BT, bool POW2>
 __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* __restrict__ filter,
                                                       uint32_t filter_bytes,
                                                       const uint32_t* __restrict__ keys, uint32_t n,
@@ -547,43 +577,46 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
                                                       uint32_t* __restrict__ tile_cnt,
                                                       uint32_t ntiles) {
   constexpr int ROUNDS = kTileKeys / BT;
-  constexpr int G = ROUNDS >= 16 ? 1 : 16 / ROUNDS;  // tiles whose keys are requested together
+  constexpr int G = ROUNDS >= 16 ? 1 : 16 / ROUNDS;  // tiles per step
   constexpr int NW = BT / 64;
+  static_assert(G * ROUNDS <= 64, "one ballot word per lane");
   extern __shared__ uint4 lds_filter[];
   __shared__ uint32_t red[G][NW];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t grid = gridDim.x;
   LdsLayout<C>::stage(lds_filter, filter, filter_bytes, tid, BT);
+  uint32_t key[G][ROUNDS];
+  lds_load_step<BT, G, ROUNDS>(keys, n, blockIdx.x, grid, tid, key);
   __syncthreads();
-  for (uint32_t t0 = blockIdx.x; t0 < ntiles; t0 += G * gridDim.x) {
-    uint32_t key[G][ROUNDS];
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-      for (int r = 0; r < ROUNDS; ++r) {
-        const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * gridDim.x;
-        const uint64_t i = t * kTileKeys + static_cast<uint64_t>(r) * BT + tid;
-        key[g][r] = load_key(keys, i < n ? i : n - 1);
-      }
+  for (uint32_t t0 = blockIdx.x; t0 < ntiles; t0 += G * grid) {
+    uint32_t next[G][ROUNDS];
+    lds_load_step<BT, G, ROUNDS>(keys, n, t0 + G * grid < ntiles ? t0 + G * grid : t0, grid, tid, next);
+    uint64_t word = 0;  // lane j: ballot word of round j of the step
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * gridDim.x;
-      const bool tv = t < ntiles;
-      const uint64_t tile_base = t * kTileKeys;
+      const uint64_t tile_base = (static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * grid) * kTileKeys;
       uint32_t cnt = 0;
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
         const uint64_t i = tile_base + static_cast<uint64_t>(r) * BT + tid;
-        const uint32_t b = block_of(key[g][r] * kMul.v[0], addr);
+        const uint32_t h0 = key[g][r] * kMul.v[0];
+        const uint32_t b = POW2 ? h0 >> addr.post : block_of(h0, addr);
         const bool hit = LdsLayout<C>::test(key[g][r], b, lds_filter) & (i < n);
         const uint64_t m = __ballot(hit);
-        if (lane == 0 && tv) store_bits(bits, (tile_base + static_cast<uint64_t>(r) * BT) / 64 + wave, m);
+        word = lane == static_cast<uint32_t>(g * ROUNDS + r) ? m : word;
         cnt += __popcll(m);
       }
       if (lane == 0) red[g][wave] = cnt;
     }
+    {
+      const uint32_t g = lane / ROUNDS, r = lane % ROUNDS;
+      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(g) * grid;
+      if (lane < static_cast<uint32_t>(G * ROUNDS) && t < ntiles)
+        store_bits(bits, (t * kTileKeys + static_cast<uint64_t>(r) * BT) / 64 + wave, word);
+    }
     __syncthreads();
     if (tid < G) {
-      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(tid) * gridDim.x;
+      const uint64_t t = static_cast<uint64_t>(t0) + static_cast<uint64_t>(tid) * grid;
       if (t < ntiles) {
         uint32_t s = 0;
 #pragma unroll
@@ -592,6 +625,10 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_lds(const typename C::word_t* 
       }
     }
     __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) key[g][r] = next[g][r];
   }
 }
 
@@ -613,12 +650,13 @@ __global__ __launch_bounds__(256) void k_bbf_build(typename C::word_t* __restric
     const uint32_t key = load_key(keys, i);
     const uint32_t b = block_of(key * kMul.v[0], addr);
     W* blk = filter + static_cast<uint64_t>(b) * C::w;
+    const HashesOf<C> H(key);
     W mask[C::sched.ngroups];
-    key_masks<C>(key, mask);
+    key_masks<C>(H, mask);
     static_for<C::sched.ngroups>([&](auto g) {
       constexpr Chunk d = C::sched.gdyn[g];
       uint32_t wi = C::sched.gword[g];
-      if constexpr (d.nbits != 0) wi += chunk_val(key, d);
+      if constexpr (d.nbits != 0) wi += H.chunk(d);
       atomic_or_word<W>(blk + wi, mask[g]);
     });
   }

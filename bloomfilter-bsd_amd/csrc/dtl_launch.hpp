@@ -25,7 +25,7 @@ inline AddrArgs make_addr(uint32_t magic, uint32_t mshift, uint32_t divisor, uin
 template <class C, int BT>
 hipError_t launch_probe_lds(const BbfProbeLaunch& L) {
   using W = typename C::word_t;
-  auto fn = k_bbf_probe_lds<C, BT>;
+  auto fn = L.addr_magic == 0 ? k_bbf_probe_lds<C, BT, true> : k_bbf_probe_lds<C, BT, false>;
   const uint32_t lds_bytes = (L.filter_bytes + 15u) & ~15u;
   hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
   if (ae != hipSuccess) return ae;
