@@ -36,6 +36,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 GPU_CLOCK_GHZ = 2.4     # MI355X peak engine clock: converts tw in GPU cycles to ns
 TW_CYCLES = (1.0, 10.0, 100.0, 1000.0)
+LDS_STAGE_MAX = 160 * 1024 - 2048   # largest filter a batch plan stages in LDS by default
 METRIC = "G probes/s on 1 MI355X + % random-access roofline; tuned vs plain config"
 
 
@@ -448,7 +449,8 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     # Two byte counts per filter f with n keys and c matches.  Block model:
     # 4 B per key, one 64 B block per key where the filter is read from
     # memory or the whole filter once per 2048-key tile where it is staged in
-    # LDS (the tuned threshold), 4 B per match.  Compulsory HBM bytes: 4 B per
+    # LDS (the plan stages filters up to the LDS capacity by default: the
+    # launches of its 32 KiB and 158 KiB size classes), 4 B per match.  Compulsory HBM bytes: 4 B per
     # key, min(filter bytes, 64 B per key) since a filter's words come from
     # HBM at most once, 4 B per match.  Filters that fit the L2 or the
     # Infinity Cache serve most block reads from there, so the block model
@@ -457,7 +459,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     hbm_bytes = 0
     for (desc, _, keys, _), c in zip(tasks, plain_counts):
         nk = keys.numel()
-        staged = desc.size_bytes <= best.lds_max_bytes
+        staged = desc.size_bytes <= LDS_STAGE_MAX
         tiles = (nk + 2047) // 2048
         block_bytes += 4 * nk + (desc.size_bytes * tiles if staged else 64 * nk) + 4 * c
         hbm_bytes += 4 * nk + min(desc.size_bytes, 64 * nk) + 4 * c

@@ -13,9 +13,11 @@
 
 using namespace dtlh;
 
-// One configuration's share of a plan: its tasks, tile table and scratch.
+// One (configuration, size class) share of a plan: its tasks, tile table and scratch.
 struct PlanGroup {
   const BbfEntry* entry = nullptr;
+  uint32_t size_class = 0;
+  uint32_t max_filter_bytes = 0;
   uint32_t n_tasks = 0;
   uint32_t total_tiles = 0;
   BatchTaskDev* d_tasks = nullptr;
@@ -28,7 +30,7 @@ struct PlanGroup {
 
 struct dtl_batch_plan {
   int device = 0;
-  std::vector<PlanGroup> groups;   // one per distinct (word_bits, w, s, z, k)
+  std::vector<PlanGroup> groups;   // one per distinct ((word_bits, w, s, z, k), size class)
 };
 
 namespace {
@@ -51,15 +53,16 @@ bool same_config(const dtl_bbf_desc& a, const dtl_bbf_desc& b) {
   return a.word_bits == b.word_bits && a.w == b.w && a.s == b.s && a.z == b.z && a.k == b.k;
 }
 
-// Builds one group from tasks that share a configuration.
+// Builds one group from tasks that share a configuration and size class.
 int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, const dtl_probe_params* params,
-                 uint32_t cus, uint32_t max_lds, PlanGroup& g) {
+                 uint32_t cus, uint32_t max_lds, uint32_t cls, PlanGroup& g) {
   const dtl_bbf_desc& d0 = tasks[ids[0]].desc;
   const BbfEntry* e = bbf_find(d0.word_bits, d0.w, d0.s, d0.z, d0.k);
   if (!e || !e->probe_batched) return DTL_EINVAL;
   const uint32_t n_tasks = static_cast<uint32_t>(ids.size());
   std::vector<BatchTaskDev> host(n_tasks);
   uint64_t tiles = 0, keys = 0;
+  uint32_t max_fb = 0;
   for (uint32_t t = 0; t < n_tasks; ++t) {
     const dtl_bbf_task& T = tasks[ids[t]];
     const dtl_bbf_desc& d = T.desc;
@@ -88,6 +91,7 @@ int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, co
       h.addr_post = 0;
     }
     h.filter_bytes = static_cast<uint32_t>(std::min<uint64_t>(d.size_bytes, 0xFFFFFFFFull));
+    max_fb = std::max(max_fb, h.filter_bytes);
     tiles += h.tile_count;
     keys += T.n;
     if (tiles > 0xFFFFFFFFull) return DTL_EINVAL;
@@ -98,6 +102,8 @@ int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, co
               tile_task.begin() + host[t].tile_begin + host[t].tile_count, t);
 
   g.entry = e;
+  g.size_class = cls;
+  g.max_filter_bytes = max_fb;
   g.n_tasks = n_tasks;
   g.total_tiles = static_cast<uint32_t>(tiles);
   g.keys = keys;
@@ -120,11 +126,15 @@ int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, co
   L.tile_cnt = g.d_tile_cnt;
   L.work_counter = g.d_tile_cnt + nt;
   const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
-  uint32_t lds_max = 32 * 1024;
+  // the class's own shape (the size of its largest filter staged, or nothing
+  // staged), unless the caller sets one
+  uint32_t lds_max = cls == 2 ? 0u : (max_fb + 15u) & ~15u;
   if (params && params->lds_max_bytes) lds_max = params->lds_max_bytes;
   if (params && (params->flags & DTL_PROBE_FORCE_GLOBAL)) lds_max = 0;
   L.lds_max_bytes = std::min(lds_max, lds_cap);
-  L.threads = (params && params->block_threads >= 1024) ? 1024 : 256;
+  uint32_t threads = (cls == 1 && d0.word_bits == 32) ? 1024u : 256u;
+  if (params && params->block_threads) threads = params->block_threads >= 1024 ? 1024u : 256u;
+  L.threads = threads;
   L.grid = params ? params->grid_blocks : 0;
   L.chunk = (params && params->chunk_tiles) ? params->chunk_tiles : 1;
   L.cu_count = cus;
@@ -153,9 +163,10 @@ void copy_shape(const BbfBatchLaunch& L, dtl_probe_params* out) {
 
 extern "C" {
 
-// Tasks may mix configurations: they are grouped by (word_bits, w, s, z, k),
-// in order of first appearance, and each group gets its own persistent launch;
-// every task's list is the same as bbf_probe's for it either way.
+// Tasks may mix configurations and filter sizes: they are grouped by
+// ((word_bits, w, s, z, k), size class), in order of first appearance, and
+// each group gets its own persistent launch; every task's list is the same as
+// bbf_probe's for it either way.
 int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
                           const dtl_probe_params* params, dtl_batch_plan** out) {
   if (!out || !tasks || n_tasks == 0) return DTL_EINVAL;
@@ -164,13 +175,16 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
   uint32_t max_lds = 0;
   int rc = current_device(&dev, &cus, &max_lds);
   if (rc != DTL_OK) return rc;
+  const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
   std::vector<std::vector<uint32_t>> ids;
-  std::vector<uint32_t> rep;  // first task of each group
+  std::vector<uint32_t> rep, cls;  // first task and size class of each group
   for (uint32_t t = 0; t < n_tasks; ++t) {
+    const uint32_t c = size_class(tasks[t].desc.size_bytes, lds_cap);
     size_t g = 0;
-    while (g < rep.size() && !same_config(tasks[rep[g]].desc, tasks[t].desc)) ++g;
+    while (g < rep.size() && !(cls[g] == c && same_config(tasks[rep[g]].desc, tasks[t].desc))) ++g;
     if (g == rep.size()) {
       rep.push_back(t);
+      cls.push_back(c);
       ids.emplace_back();
     }
     ids[g].push_back(t);
@@ -179,7 +193,7 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
   p->device = dev;
   p->groups.resize(ids.size());
   for (size_t g = 0; g < ids.size(); ++g) {
-    rc = group_create(tasks, ids[g], params, static_cast<uint32_t>(cus), max_lds, p->groups[g]);
+    rc = group_create(tasks, ids[g], params, static_cast<uint32_t>(cus), max_lds, cls[g], p->groups[g]);
     if (rc != DTL_OK) {
       plan_free(p);
       return rc;
@@ -208,12 +222,13 @@ uint32_t bbf_batch_plan_group_count(const dtl_batch_plan* p) {
 
 void bbf_batch_plan_destroy(dtl_batch_plan* p) { plan_free(p); }
 
-// Autotuner: for each configuration group, times every candidate launch
-// shape of the persistent probe kernel on the group's own tasks (median of
-// `reps` runs, HIP events on `stream`) and keeps the fastest.  Candidates:
-// LDS staging threshold {0, 8, 16, 32, 64, 96, 150 KiB} x workgroup {256,
-// 1024} threads x tiles per grab {1, 2, 4, 8, 16}; the grid is CUs x
-// occupancy.  *best_out receives the shape of the group with the most keys.
+// Autotuner: for each group, times every candidate launch shape of the
+// persistent probe kernel on the group's own tasks (median of `reps` runs,
+// HIP events on `stream`) and keeps the fastest.  Candidates: LDS staging
+// threshold {0, the group's largest filter} x workgroup {256, 1024} threads x
+// tiles per grab {1, 2, 4, 8, 16} x grid {CUs x occupancy, half of it (at
+// least one workgroup per CU)}.  *best_out receives the shape of the group
+// with the most keys.
 int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
                         dtl_probe_params* best_out) {
   if (!p || reps == 0) return DTL_EINVAL;
@@ -224,7 +239,6 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
   if (rc != DTL_OK) return rc;
   (void)hipGetLastError();
   const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0;
-  const uint32_t lds_opts[] = {0, 8 << 10, 16 << 10, 32 << 10, 64 << 10, 96 << 10, 150 << 10};
   const uint32_t thr_opts[] = {256, 1024};
   const uint32_t chunk_opts[] = {1, 2, 4, 8, 16};
   hipEvent_t a = nullptr, b = nullptr;
@@ -239,14 +253,28 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
     const BbfBatchLaunch saved = g.launch;
     float best_ms = 1e30f;
     BbfBatchLaunch best = saved;
-    for (uint32_t lds : lds_opts) {
-      if (lds > lds_cap || rc != DTL_OK) continue;
+    const uint32_t staged = (g.max_filter_bytes + 15u) & ~15u;
+    const uint32_t lds_opts[] = {0u, staged <= lds_cap ? staged : 0u};
+    for (uint32_t li = 0; li < 2; ++li) {
+      const uint32_t lds = lds_opts[li];
+      if (rc != DTL_OK || (li == 1 && lds == 0)) continue;
       for (uint32_t thr : thr_opts) {
+        // the full grid (CUs x occupancy) and half of it
+        uint32_t full = 0;
+        {
+          BbfBatchLaunch probe = g.launch;
+          probe.lds_max_bytes = lds;
+          probe.threads = thr;
+          full = g.entry->batched_grid(probe);
+        }
+        const uint32_t grid_opts[] = {full, std::max(static_cast<uint32_t>(cus), full / 2)};
+        for (uint32_t gi2 = 0; gi2 < 2 && rc == DTL_OK; ++gi2) {
+          if (gi2 == 1 && grid_opts[1] >= grid_opts[0]) continue;
         for (uint32_t ch : chunk_opts) {
           g.launch.lds_max_bytes = lds;
           g.launch.threads = thr;
           g.launch.chunk = ch;
-          g.launch.grid = 0;
+          g.launch.grid = grid_opts[gi2];
           std::vector<float> t;
           for (uint32_t r = 0; r < reps + 1 && rc == DTL_OK; ++r) {
             if (hipEventRecord(a, st) != hipSuccess || group_run(g, st) != hipSuccess ||
@@ -265,6 +293,7 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
             best_ms = med;
             best = g.launch;
           }
+        }
         }
       }
     }

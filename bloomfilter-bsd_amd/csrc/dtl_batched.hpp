@@ -118,6 +118,26 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
   }
 }
 
+// CUs x workgroups per CU of the launch shape (LDS reservation, workgroup size).
+template <class C, int BT>
+uint32_t batched_full_grid_bt(const BbfBatchLaunch& L) {
+  auto fn = k_bbf_probe_batched<C, BT>;
+  const uint32_t lds_bytes = (L.lds_max_bytes + 15u) & ~15u;
+  int per_cu = 0;
+  if (allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BT, lds_bytes) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(L.cu_count > 0 ? L.cu_count : 1);
+}
+
+template <class C>
+uint32_t batched_full_grid(const BbfBatchLaunch& L) {
+  if constexpr (sizeof(typename C::word_t) == 4) {
+    if (L.threads == 1024) return batched_full_grid_bt<C, 1024>(L);
+  }
+  return batched_full_grid_bt<C, 256>(L);
+}
+
 template <class C, int BT>
 hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_counter, uint32_t chunk) {
   auto fn = k_bbf_probe_batched<C, BT>;
@@ -125,12 +145,7 @@ hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_count
   hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
   if (ae != hipSuccess) return ae;
   uint32_t grid = L.grid;
-  if (grid == 0) {
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BT, lds_bytes);
-    if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    grid = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(L.cu_count > 0 ? L.cu_count : 1);
-  }
+  if (grid == 0) grid = batched_full_grid_bt<C, BT>(L);
   hipLaunchKernelGGL(fn, dim3(grid), dim3(BT), lds_bytes, L.stream, L.tasks, L.tile_task,
                      L.total_tiles, chunk, work_counter, L.lds_max_bytes, L.bits, L.tile_cnt);
   return hipGetLastError();

@@ -129,12 +129,30 @@ inline uint32_t hit_bucket_shift(uint32_t ntiles) {
   return s;
 }
 
+// Size classes of a plan's filters.  Each (configuration, class) pair is its
+// own persistent launch, because the LDS a launch reserves per workgroup
+// decides its occupancy:
+//   0: up to kSmallFilterBytes   staged in LDS, 256-thread workgroups, several per CU;
+//   1: up to the LDS capacity    staged in LDS, 1024-thread workgroups (32-bit
+//                                words; 256 for 64-bit words), one per CU;
+//   2: larger                    read from L2 / Infinity Cache / HBM, lane-split
+//                                where the configuration allows.
+// Before the split, one launch per configuration reserved 32 KiB per
+// workgroup (the tuned compromise), so filters of 32-158 KiB were gathered
+// from L2 like the large ones.
+constexpr uint32_t kSmallFilterBytes = 32 * 1024;
+
+inline uint32_t size_class(uint64_t filter_bytes, uint32_t lds_cap) {
+  return filter_bytes <= kSmallFilterBytes ? 0u : filter_bytes <= lds_cap ? 1u : 2u;
+}
+
 struct BbfEntry {
   uint32_t word_bits, w, s, z, k;
   hipError_t (*probe)(const BbfProbeLaunch&);
   hipError_t (*build)(const BbfBuildLaunch&);
   hipError_t (*probe_batched)(const BbfBatchLaunch&);  // may be null
   hipError_t (*probe_part)(const BbfPartLaunch&);      // probe phase of the partitioned path
+  uint32_t (*batched_grid)(const BbfBatchLaunch&);     // CUs x occupancy of a batched launch shape
   uint32_t split_lanes;                                // lanes per key of the lane-split kernel, 0 = none
 };
 

@@ -109,6 +109,7 @@ BbfEntry make_entry() {
   e.build = &launch_build<C>;
   e.probe_batched = &launch_probe_batched<C>;
   e.probe_part = &launch_probe_part<C>;
+  e.batched_grid = &batched_full_grid<C>;
   e.split_lanes = C::split.ok ? static_cast<uint32_t>(C::split.lanes) : 0u;
   return e;
 }

@@ -157,7 +157,7 @@ std::vector<double> candidate_m(const dtl_cost_model& M, const Cfg& c, uint64_t 
 }
 
 constexpr uint32_t kCalTasks = 32;
-constexpr uint32_t kCalPlanLds = 32 * 1024, kCalPlanThreads = 256, kCalPlanChunk = 4;
+constexpr uint32_t kCalPlanChunk = 4;
 constexpr double kCalTargetMs = 0.5;     // aimed duration of a timed run
 constexpr uint64_t kCalEstKeys = 1ull << 17;  // keys per task of the estimation run (at most)
 
@@ -165,8 +165,10 @@ constexpr uint64_t kCalEstKeys = 1ull << 17;  // keys per task of the estimation
 // runs: kCalTasks tasks, each probing its own zero filter of the size (tasks
 // share one filter from 64 MiB on) with K absent keys (windows of `keys` that
 // start at evenly spaced offsets), run by the persistent batched kernel at the
-// plan's standard launch shape (LDS staging up to 32 KiB, 256-thread
-// workgroups, 4 tiles per grab) with the per-task compaction.
+// plan's default launch shape for the filter's size class (size_class: the
+// filter staged in LDS by 256-thread workgroups up to 32 KiB and by
+// 1024-thread ones (32-bit words) up to the LDS capacity, read from memory
+// above), 4 tiles per grab, with the per-task compaction.
 // Per size, all configurations are timed in one enqueued sequence (no host
 // round trip between runs, so the GPU stays busy and its clock steady), in
 // `passes` passes that alternate the order; a configuration's cost per key is
@@ -228,8 +230,9 @@ int calibrate_plans(dtl_cost_model& M, const uint32_t* keys, uint64_t nkeys, uin
   L.bits = static_cast<uint64_t*>(bits.p);
   L.tile_cnt = static_cast<uint32_t*>(tcnt.p);
   L.work_counter = static_cast<uint32_t*>(tcnt.p) + tiles_max;
-  L.lds_max_bytes = std::min(kCalPlanLds, max_lds > 2048 ? max_lds - 2048 : 0u);
-  L.threads = kCalPlanThreads;
+  const uint32_t lds_cap = max_lds > 2048 ? max_lds - 2048 : 0u;
+  L.lds_max_bytes = 0;
+  L.threads = 256;
   L.grid = 0;
   L.chunk = kCalPlanChunk;
   L.cu_count = cus;
@@ -275,6 +278,9 @@ int calibrate_plans(dtl_cost_model& M, const uint32_t* keys, uint64_t nkeys, uin
     uint32_t lvl = 0;
     while ((kTileKeys << lvl) < K) ++lvl;
     L.tasks = static_cast<const BatchTaskDev*>(tasks.p) + (static_cast<size_t>(c) * 2 + slot) * kCalTasks;
+    const uint32_t cls = size_class(desc[c].size_bytes, lds_cap);
+    L.lds_max_bytes = cls == 2 ? 0u : static_cast<uint32_t>((desc[c].size_bytes + 15u) & ~15ull);
+    L.threads = (cls == 1 && desc[c].word_bits == 32) ? 1024u : 256u;
     L.tile_task = static_cast<const uint32_t*>(tt.p) + map_at[lvl];
     L.total_tiles = static_cast<uint32_t>(kCalTasks * (K / kTileKeys));
     hipError_t he = zero_u32(L.work_counter, st);
@@ -414,13 +420,8 @@ int dtl_model_calibrate(const dtl_calib_params* p, dtl_cost_model** out, dtl_str
       if (rc == DTL_OK && hipStreamSynchronize(st) != hipSuccess) rc = DTL_EHIP;
     }
   }
-  // lookup cost per (config, size), in a batch plan: kCalTasks tasks, each
-  // probing its own zero filter of that size (every key a negative probe, the
-  // common case; tasks share one filter from 64 MiB on), run by the
-  // persistent batched kernel at the plan's standard launch shape (LDS
-  // staging up to 32 KiB, 256-thread workgroups, 4 tiles per grab) with the
-  // per-task compaction.  The cost per key is the slope between runs of K and
-  // 2K keys per task, so the fixed cost of the launches drops out.
+  // lookup cost per (config, size), in batch plans of zero filters (every
+  // key a negative probe, the common case): calibrate_plans
   if (rc == DTL_OK) rc = calibrate_plans(*M, static_cast<uint32_t*>(keys.p), nkeys, reps, st);
   // false-positive rate per (config, bits per key), measured on real filters
   if (rc == DTL_OK && gen_keys(static_cast<uint32_t*>(absent.p), nfpr, 4242, 1, st) != hipSuccess) rc = DTL_EHIP;

@@ -702,7 +702,12 @@ def test_batched_mixed_configurations():
         tasks.append((desc, words, to_dev(probe), 7 * i))
         expect.append(f.batch_contains(oracle_words(f, build), probe, 7 * i))
     plan = D.BatchPlan(tasks)
-    assert plan.group_count() == len(MIXED_CFGS)
+    # one launch per (configuration, size class): filters up to 32 KiB, up to
+    # the LDS capacity (160 KiB less 2 KiB), larger
+    cls = lambda nb: 0 if nb <= 32 * 1024 else 1 if nb <= 160 * 1024 - 2048 else 2
+    groups = {((t[0].word_bits, t[0].w, t[0].s, t[0].z, t[0].k), cls(t[0].size_bytes)) for t in tasks}
+    assert len({g[0] for g in groups}) == len(MIXED_CFGS) and len({g[1] for g in groups}) > 1
+    assert plan.group_count() == len(groups)
     plan.tune(reps=1)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
