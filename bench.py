@@ -542,7 +542,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     return res
 
 
-def run_learned(torch, D, train_fraction=0.25, samples=2000000, epochs=32, model_path=None):
+def run_learned(torch, D, train_fraction=1.0, samples=2000000, epochs=32, model_path=None):
     """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
     bits, timed on the GPU; FPR measured per configuration and bits per key),
     MLP trained on a seeded `train_fraction` of the (configuration, size)
@@ -738,7 +738,7 @@ def main():
     ap.add_argument("--learned", action="store_true",
                     help="only run the learned configuration model; with --batched also the learned + batched "
                          "semi-join")
-    ap.add_argument("--train-fraction", type=float, default=0.25)
+    ap.add_argument("--train-fraction", type=float, default=1.0)
     ap.add_argument("--model-path", default=None, help="save the calibrated + trained model here")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates HBM byte counters)")

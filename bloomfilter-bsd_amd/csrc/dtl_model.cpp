@@ -488,7 +488,9 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 }
 
 // Trains the MLP on a seeded `train_fraction` of the calibrated (config,
-// size) cells (the rest of the table is never seen by the MLP).  Rows: a random cell,
+// size) cells (the rest of the table is never seen by the MLP; with 1, the
+// default of the benchmark, on all of them, and then at sizes between the
+// calibrated ones too, see draw_rows).  Rows: a random cell,
 // bits per key log-uniform in [4, 20] (the candidate range) and tw
 // log-uniform in [kTwMinNs, kTwMaxNs], target y = log(overhead) standardised.  The
 // loss weights a row by exp(-(y - y_best) / kTemp) + kFloor, y_best = the
@@ -498,9 +500,10 @@ double dtl_model_fpr(const dtl_cost_model* M, int cfg, double bits_per_key) {
 // to be tight, and a poor candidate predicted to be good is what makes a wrong
 // choice.  The MLP (two hidden layers of 64) cannot fit the whole table to a
 // few percent, so it spends its capacity there.  kRestarts networks (different initialisations) are trained
-// and the one whose choices have the lowest mean regret on a query grid,
-// measured on the training cells only (candidates at calibrated sizes, table
-// cost of those cells), is kept.
+// and the one whose choices have the lowest mean regret on a query grid is
+// kept: with every cell calibrated, over the candidates dtl_model_choose
+// takes (interpolated table cost); otherwise over the training cells only
+// (candidates at calibrated sizes, table cost of those cells).
 constexpr int kRestarts = 4;
 // Row weights: exp(-(y - y_best) / kTemp) + kFloor from the table, plus
 // kDyn * exp(-max(prediction - y_best, 0) / kTemp) during training (rows the
@@ -517,9 +520,14 @@ struct Rows {
   std::vector<float> X, Y, W, B;  // features, log overhead, weight, log of the best overhead of the row's query
 };
 
-// Draws `count` rows from `cells`; y_best over every cell in `ref`.
+// Draws `count` rows from `cells`; y_best over every cell in `ref`.  With
+// `between` (every cell calibrated), a row's filter size is drawn uniformly
+// in log2 m within half a calibrated step of its cell and its lookup time
+// interpolated like the table's (interp_t): the choices are made at sizes
+// m = n * bits per key, which mostly fall between calibrated sizes, and a
+// network trained only at those leaves the space between them unfitted.
 void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std::vector<uint8_t>& ref,
-               uint32_t count, uint64_t& s, Rows& r) {
+               uint32_t count, uint64_t& s, Rows& r, bool between) {
   const int nc = static_cast<int>(M.cfgs.size()), ns = M.nsizes();
   auto next = [&]() {
     s ^= s << 13;
@@ -533,16 +541,20 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
   r.W.assign(count, 0.f);
   r.B.assign(count, 0.f);
   std::vector<int> s_si(count);
-  std::vector<double> s_bpk(count), s_tw(count);
+  std::vector<double> s_l2m(count), s_bpk(count), s_tw(count);
   for (uint32_t i = 0; i < count; ++i) {
     const int cell = cells[next() % cells.size()];
     const int c = cell / ns, si = cell % ns;
-    const double m = std::ldexp(1.0, M.log2_min + si);
+    double l2m = M.log2_min + si;
+    if (between) l2m = std::min<double>(M.log2_max, std::max<double>(M.log2_min, l2m + unif() - 0.5));
+    const double m = std::exp2(l2m);
     const double bpk = 4.0 * std::pow(5.0, unif());
     const double tw = kTwMinNs * std::pow(kTwMaxNs / kTwMinNs, unif());
     features(M, c, m, m / bpk, tw, &r.X[static_cast<size_t>(i) * kFeat]);
-    r.Y[i] = static_cast<float>(std::log(M.t_ns[cell] + tw * interp_fpr(M, c, bpk)));
+    const double t = between ? interp_t(M, c, l2m) : M.t_ns[cell];
+    r.Y[i] = static_cast<float>(std::log(t + tw * interp_fpr(M, c, bpk)));
     s_si[i] = si;
+    s_l2m[i] = l2m;
     s_bpk[i] = bpk;
     s_tw[i] = tw;
   }
@@ -554,7 +566,9 @@ void draw_rows(const dtl_cost_model& M, const std::vector<int>& cells, const std
         double best = 1e300;
         for (int c = 0; c < nc; ++c) {
           const size_t cell = static_cast<size_t>(c) * ns + s_si[i];
-          if (ref[cell]) best = std::min(best, M.t_ns[cell] + s_tw[i] * interp_fpr(M, c, s_bpk[i]));
+          if (!ref[cell]) continue;
+          const double t = between ? interp_t(M, c, s_l2m[i]) : M.t_ns[cell];
+          best = std::min(best, t + s_tw[i] * interp_fpr(M, c, s_bpk[i]));
         }
         r.B[i] = static_cast<float>(std::log(best));
         r.W[i] = static_cast<float>(std::exp(-(r.Y[i] - r.B[i]) / kTemp) + kFloor);
@@ -588,8 +602,9 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     if (M->train_cell[i]) fit_cells.push_back(i);
   }
   if (fit_cells.empty()) return DTL_EINVAL;
+  const bool all_cells = fit_cells.size() == static_cast<size_t>(nc) * ns;
   Rows fit;
-  draw_rows(*M, fit_cells, M->train_cell, samples, s, fit);
+  draw_rows(*M, fit_cells, M->train_cell, samples, s, fit, all_cells);
   double sum = 0, sum2 = 0;
   for (float y : fit.Y) {
     sum += y;
@@ -599,8 +614,10 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
   M->y_std = std::sqrt(std::max(1e-12, sum2 / samples - M->y_mean * M->y_mean));
   for (auto& v : fit.Y) v = static_cast<float>((v - M->y_mean) / M->y_std);
   for (auto& v : fit.B) v = static_cast<float>((v - M->y_mean) / M->y_std);
-  // selection queries: n = 2^10 .. 2^28, tw = 0.42 .. 1000 ns; candidates are
-  // the training cells whose size gives 4..20 bits per key
+  // selection queries: n = 2^10 .. 2^28, tw = 0.42 .. 1000 ns.  With every
+  // cell calibrated the candidates are those of dtl_model_choose (m = n x
+  // 4..20 bits per key, table cost interpolated); otherwise the training
+  // cells whose size gives 4..20 bits per key
   struct Query {
     uint32_t begin, end;
   };
@@ -611,16 +628,25 @@ int dtl_model_train(dtl_cost_model* M, double train_fraction, uint32_t samples, 
     for (double tw : {0.42, 1.0, 3.0, 10.0, 30.0, 100.0, 300.0, 1000.0}) {
       Query q{static_cast<uint32_t>(qcost.size()), 0};
       const double n = std::ldexp(1.0, ln);
-      for (int c = 0; c < nc; ++c)
+      float f[kFeat];
+      for (int c = 0; c < nc; ++c) {
+        if (all_cells) {
+          for (double m : candidate_m(*M, M->cfgs[c], static_cast<uint64_t>(n))) {
+            features(*M, c, m, n, tw, f);
+            qx.insert(qx.end(), f, f + kFeat);
+            qcost.push_back(table_cost(*M, c, m, n, tw));
+          }
+          continue;
+        }
         for (int si = 0; si < ns; ++si) {
           if (!M->train_cell[static_cast<size_t>(c) * ns + si]) continue;
           const double m = std::ldexp(1.0, M->log2_min + si), bpk = m / n;
           if (bpk < 4.0 || bpk > 20.0) continue;
-          float f[kFeat];
           features(*M, c, m, n, tw, f);
           qx.insert(qx.end(), f, f + kFeat);
           qcost.push_back(M->t_ns[static_cast<size_t>(c) * ns + si] + tw * interp_fpr(*M, c, bpk));
         }
+      }
       q.end = static_cast<uint32_t>(qcost.size());
       if (q.end > q.begin + 1) queries.push_back(q);
     }
