@@ -714,7 +714,67 @@ def load_traffic(name):
         return json.load(fh)
 
 
+def summarize_traffic(fetch_csv, write_csv, commit):
+    """HBM bytes of the headline call from two `rocprofv3 --pmc` passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) of `bench.py --headline-only --calib-copy`:
+    each counter is scaled by true bytes / counter bytes of the 1 GiB device copy
+    that --calib-copy runs first; the phase is the last partitioned call
+    (k_fill_zero_u32 .. k_hit_bits), the direct kernel its last launch."""
+    import csv
+
+    def rows(path, counter):
+        out = []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"]) * 1024.0))
+        return sorted(out)
+
+    f, w = rows(fetch_csv, "FETCH_SIZE"), rows(write_csv, "WRITE_SIZE")
+    if [x[1] for x in f] != [x[1] for x in w]:
+        raise SystemExit("the two passes dispatched different kernels")
+    copy_bytes = 1 << 30
+    ci = next(i for i, x in enumerate(f) if x[1].startswith("__amd_rocclr_copyBuffer"))
+    fs, ws = copy_bytes / f[ci][2], copy_bytes / w[ci][2]
+
+    def short(name):
+        name = name.split("(")[0].replace("void ", "").replace("dtlh::", "")
+        return name.replace("unsigned long", "u64")
+
+    hist = [i for i, x in enumerate(f) if "k_part_hist<" in x[1]][-1]
+    first = max(i for i in range(hist) if "k_fill_zero_u32" in f[i][1])
+    last = next(i for i in range(hist, len(f)) if "k_hit_bits" in f[i][1])
+    kern = [{"kernel": short(f[i][1]), "fetch_bytes": round(f[i][2] * fs), "write_bytes": round(w[i][2] * ws)}
+            for i in range(first, last + 1)]
+    fb, wb = sum(k["fetch_bytes"] for k in kern), sum(k["write_bytes"] for k in kern)
+    d = [i for i, x in enumerate(f) if "k_bbf_probe_global<" in x[1]][-1]
+    return {
+        "workload": "cache-sectorized BBF u64 w8 s8 k8, 1 GiB filter, 2^28 probe keys at 1% hits (bench.py defaults)",
+        "commit": commit,
+        "commands": ["rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pf -o pf --output-format csv -- "
+                     "python3 bench.py --steps 2 --warmup 1 --no-cpu --headline-only --calib-copy",
+                     "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pw -o pw --output-format csv -- "
+                     "python3 bench.py --steps 2 --warmup 1 --no-cpu --headline-only --calib-copy",
+                     "python3 bench.py --summarize-traffic gpurun_out/pf/pf_counter_collection.csv "
+                     "gpurun_out/pw/pw_counter_collection.csv --commit <measured commit>"],
+        "copy_calibration": {"copy_bytes": copy_bytes, "fetch_kib": f[ci][2] / 1024, "write_kib": w[ci][2] / 1024,
+                             "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)},
+        "partitioned_phase": {"kernels": kern, "fetch_bytes": fb, "write_bytes": wb},
+        "bytes_per_launch": fb + wb,
+        "direct_kernel": {"kernel": short(f[d][1]), "fetch_bytes": round(f[d][2] * fs),
+                          "write_bytes": round(w[d][2] * ws),
+                          "bytes_per_launch": round(f[d][2] * fs) + round(w[d][2] * ws)},
+        "algorithmic_bytes_per_launch": (4.0 + 64.0) * (1 << 28) + 4.0 * 2763103,
+    }
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-traffic":
+        sp = argparse.ArgumentParser()
+        sp.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+        sp.add_argument("--commit", default=None)
+        a = sp.parse_args()
+        print(json.dumps(summarize_traffic(a.summarize_traffic[0], a.summarize_traffic[1], a.commit), indent=1))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
