@@ -485,12 +485,27 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
+    // Bins of fixed capacity (no histogram pass) where they hold 4 tiles of
+    // keys or more: the expected share + 3% + 256 keys (a bin's count has a
+    // standard deviation of sqrt(expected), 0.4% at the 2^28-key benchmark),
+    // whole tiles; keys past a full bin go to other bins.  Smaller batches,
+    // batches of 2^31 keys or more, and DTL_PROBE_PART_HISTOGRAM size the
+    // partitions with a histogram pass instead.
+    uint32_t bin_cap = 0;
+    if (!(flags & DTL_PROBE_PART_HISTOGRAM) && n < (1ull << 31) && n >= nbins * 4ull * kTileKeys) {
+      const uint64_t want = (n + nbins - 1) / nbins * 103 / 100 + 256;
+      bin_cap = static_cast<uint32_t>((want + kTileKeys - 1) / kTileKeys * kTileKeys);
+    }
+    const uint32_t ptiles = bin_cap ? static_cast<uint32_t>(nbins * bin_cap / kTileKeys) : ntiles;
+    const size_t pbits_bytes = static_cast<size_t>(ptiles) * kTileWords * 8;
+    const size_t pcnt_r = up((static_cast<size_t>(ptiles) + kScanTmpWords) * 4);
     const size_t part_bytes = up((nbins * 3 + 1) * 4);
     const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
     const size_t matrix_bytes = up(static_cast<size_t>(hit_groups) * nbuckets * 4);
-    const size_t pair_bytes = up(static_cast<size_t>(n) * 4);
+    const size_t pair_bytes = up(static_cast<size_t>(ptiles) * kTileKeys * 4);
     void* scratch = nullptr;
-    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + matrix_bytes + 2 * pair_bytes,
+    rc = get_scratch(dev, st, bits_bytes + cnt_r + pbits_bytes + pcnt_r + part_bytes + bucket_bytes + matrix_bytes +
+                                  2 * pair_bytes,
                      &scratch);
     if (rc != DTL_OK) return rc;
     char* sp = static_cast<char*>(scratch);
@@ -507,13 +522,15 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     P.tile_cnt = reinterpret_cast<uint32_t*>(sp);
     sp += cnt_r;
     P.pbits = reinterpret_cast<uint64_t*>(sp);
-    sp += bits_bytes;
+    sp += pbits_bytes;
     P.ptile_cnt = reinterpret_cast<uint32_t*>(sp);
-    sp += cnt_r;
+    sp += pcnt_r;
     P.counts = reinterpret_cast<uint32_t*>(sp);
     P.offsets = P.counts + nbins;
     P.cursor = P.offsets + nbins + 1;
     sp += part_bytes;
+    P.bin_cap = bin_cap;
+    P.ptiles = ptiles;
     P.d_hits = reinterpret_cast<uint64_t*>(sp);
     P.bucket_shift = bucket_shift;
     P.nbuckets = nbuckets;
@@ -728,6 +745,7 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     sp += hit_bytes;
     uint32_t* hp = reinterpret_cast<uint32_t*>(sp);  // matches of both passes (<= 2n)
     P.ntiles = ntiles;
+    P.ptiles = ntiles;  // partitions sized by histograms (bin_cap 0)
     P.split = 0;
     P.cu_count = di.cu_count;
     P.stream = st;

@@ -85,6 +85,13 @@ struct BbfPartLaunch {
   uint32_t cf_which;         // cuckoo probe pass: 1 = first bucket + victim, 2 = alternate bucket
   uint64_t cf_victim_offset;
   uint32_t nparts;
+  // 0: partitions sized by a histogram pass (offsets from its scan).
+  // Otherwise every (partition, lane) bin has this fixed capacity (a
+  // multiple of kTileKeys): no histogram pass; a key whose bin is full takes
+  // a slot of another bin (probe results do not depend on where a key sits,
+  // only locality does), and cursor[] ends as each bin's fill.
+  uint32_t bin_cap;
+  uint32_t ptiles;        // tiles of the grouped order (ntiles, or bins * bin_cap / kTileKeys)
   uint32_t scatter_threads;  // 256, 512 or 1024 (tile = 16 keys per thread)
   uint32_t* counts;       // nparts * kPartLanes bins
   // cuckoo probe: pass 1 also fills the alternate-bucket bins (counts_next,
@@ -94,10 +101,10 @@ struct BbfPartLaunch {
   int counts_ready;
   uint32_t* offsets;      // bins + 1
   uint32_t* cursor;       // bins
-  uint32_t* keys_p;       // n keys grouped by partition (later: hit positions)
+  uint32_t* keys_p;       // keys grouped by partition, ptiles * kTileKeys slots (later: hit positions)
   uint32_t* pos_p;        // their original positions (later: hit positions by bucket)
-  uint64_t* pbits;        // match bitmap in partitioned order, ntiles * kTileWords words
-  uint32_t* ptile_cnt;    // ntiles
+  uint64_t* pbits;        // match bitmap in partitioned order, ptiles * kTileWords words
+  uint32_t* ptile_cnt;    // ptiles (+ kScanTmpWords)
   uint64_t* d_hits;       // device: number of matches
   uint32_t bucket_shift;  // positions per bucket of the hit sort: 2^bucket_shift
   uint32_t nbuckets;      // position buckets (<= kHitBucketsMax)

@@ -188,6 +188,26 @@ __device__ __forceinline__ uint32_t block_sum_waves(uint32_t v, uint32_t* red) {
 // the grouped keys and the filter slices of that range are cached in its
 // own L2 one after another (grid = 8 * tiles_per_xcd).
 // ---------------------------------------------------------------------------
+// Keys grouped in bins of tiles_per_bin tiles (the partitioned probe of
+// blocked Bloom filters): bin_fill[b] is the final cursor of bin b (an index
+// of the grouped array, possibly past the bin's capacity); keys of the bin
+// occupy [b * cap, min(bin_fill[b], (b + 1) * cap)).  Returns that end for
+// the bin of `tile`.
+__device__ __forceinline__ uint32_t bin_end_of(const uint32_t* __restrict__ bin_fill, uint32_t tile,
+                                               uint32_t tiles_per_bin) {
+  const uint32_t b = tile / tiles_per_bin;
+  const uint32_t cap_end = (b + 1u) * tiles_per_bin * kTileKeys;
+  const uint32_t f = bin_fill[b];
+  return f < cap_end ? f : cap_end;
+}
+
+// A tile with no keys: its ballot words and count are zero.
+__device__ __forceinline__ void clear_tile(uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
+                                           uint32_t tile) {
+  if (threadIdx.x < kTileWords) bits[static_cast<uint64_t>(tile) * kTileWords + threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) tile_cnt[tile] = 0u;
+}
+
 #ifndef DTL_GLOBAL_UNROLL
 #define DTL_GLOBAL_UNROLL 8  // keys in flight per lane (build-time tuning knob; 8 measured fastest)
 #endif
@@ -199,7 +219,7 @@ template <class C>
 __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
     const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
     AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
-    uint32_t tiles_per_xcd, uint32_t ntiles) {
+    uint32_t tiles_per_xcd, uint32_t ntiles, const uint32_t* __restrict__ bin_fill, uint32_t tiles_per_bin) {
   using W = typename C::word_t;
   constexpr int BT = kGlobalThreads;
   constexpr int ROUNDS = kTileKeys / BT;
@@ -210,6 +230,13 @@ __global__ __launch_bounds__(kGlobalThreads) void k_bbf_probe_global(
   if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint64_t tile_base = static_cast<uint64_t>(tile) * kTileKeys;
+  if (bin_fill) {  // keys grouped in bins of fixed capacity: this tile's bin ends at its fill
+    n = bin_end_of(bin_fill, tile, tiles_per_bin);
+    if (n <= tile_base) {  // nothing of the bin in this tile (uniform, before any barrier)
+      clear_tile(bits, tile_cnt, tile);
+      return;
+    }
+  }
   uint32_t cnt = 0;
 #pragma unroll
   for (int r0 = 0; r0 < ROUNDS; r0 += U) {
@@ -445,12 +472,19 @@ template <class C>
 __global__ __launch_bounds__(kGlobalThreads, DTL_SPLIT_WAVES) void k_bbf_probe_split(
     const typename C::word_t* __restrict__ filter, const uint32_t* __restrict__ keys, uint32_t n,
     AddrArgs addr, uint64_t* __restrict__ bits, uint32_t* __restrict__ tile_cnt,
-    uint32_t tiles_per_xcd, uint32_t ntiles) {
+    uint32_t tiles_per_xcd, uint32_t ntiles, const uint32_t* __restrict__ bin_fill, uint32_t tiles_per_bin) {
   constexpr int BT = kGlobalThreads;
   __shared__ uint32_t red[BT / 64];
   const uint32_t tile =
       tiles_per_xcd ? (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3) : blockIdx.x;
   if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  if (bin_fill) {  // keys grouped in bins of fixed capacity: this tile's bin ends at its fill
+    n = bin_end_of(bin_fill, tile, tiles_per_bin);
+    if (n <= tile * kTileKeys) {  // nothing of the bin in this tile (uniform, before any barrier)
+      clear_tile(bits, tile_cnt, tile);
+      return;
+    }
+  }
   const SplitLane<C> sl(threadIdx.x & (C::split.lanes - 1));
   const uint32_t cnt = split_probe_tile<C, BT>(sl, filter, keys, n, addr, static_cast<uint64_t>(tile) * kTileKeys,
                                                bits + static_cast<uint64_t>(tile) * kTileWords);

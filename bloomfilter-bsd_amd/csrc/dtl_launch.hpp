@@ -64,24 +64,26 @@ hipError_t launch_probe(const BbfProbeLaunch& L) {
   hipLaunchKernelGGL(fn, dim3(L.ntiles), dim3(kGlobalThreads), 0, L.stream,
                      static_cast<const W*>(L.words), L.keys, L.n,
                      make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post),
-                     L.bits, L.tile_cnt, 0u, L.ntiles);
+                     L.bits, L.tile_cnt, 0u, L.ntiles, static_cast<const uint32_t*>(nullptr), 0u);
   return hipGetLastError();
 }
 
 // Probe phase of the partitioned path: the global (or lane-split) kernel
-// over the grouped keys, XCD-ordered, into the partitioned-order bitmap.
+// over the grouped keys (L.ptiles tiles; with capacity bins, each tile's
+// keys end at its bin's fill), XCD-ordered, into the partitioned-order bitmap.
 template <class C>
 hipError_t launch_probe_part(const BbfPartLaunch& L) {
   using W = typename C::word_t;
-  const uint32_t per_xcd = (L.ntiles + 7u) / 8u;
+  const uint32_t per_xcd = (L.ptiles + 7u) / 8u;
   auto fn = k_bbf_probe_global<C>;
   if constexpr (C::split.ok) {
     if (L.split) fn = k_bbf_probe_split<C>;
   }
+  const uint32_t* fill = L.bin_cap ? L.cursor : nullptr;
   hipLaunchKernelGGL(fn, dim3(per_xcd * 8u), dim3(kGlobalThreads), 0, L.stream,
                      static_cast<const W*>(L.words), L.keys_p, L.n,
                      make_addr(L.addr_magic, L.addr_mshift, L.addr_divisor, L.addr_post), L.pbits,
-                     L.ptile_cnt, per_xcd, L.ntiles);
+                     L.ptile_cnt, per_xcd, L.ptiles, fill, L.bin_cap / kTileKeys);
   return hipGetLastError();
 }
 

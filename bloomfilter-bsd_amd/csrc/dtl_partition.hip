@@ -218,19 +218,37 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
 // workgroups fit on a CU: one loads and sorts while the other writes (the
 // launch bound asks for 2 workgroups' waves per SIMD, <= 64 VGPRs at 1024
 // threads).
-template <int kScatterThreads, int KIND>
+// Slot for a key that did not fit in its full bin (capacity bins): one slot
+// reserved at a time from bins probed from a hash-chosen start, so every
+// reserved slot below a bin's capacity is written.  The bins' capacities add
+// up to at least n, so a free slot exists; the bound on the walk only keeps
+// the loop finite.
+__device__ __forceinline__ uint32_t spill_slot(uint32_t* __restrict__ cursor, uint32_t nbins, uint32_t cap,
+                                               uint32_t key, uint32_t pos) {
+  uint32_t b = ((key ^ (pos * 0x9E3779B9u)) * 0x85EBCA6Bu) % nbins;
+  for (uint32_t t = 0; t < 2u * nbins; ++t) {
+    const uint32_t s = atomicAdd(&cursor[b], 1u);
+    if (s < (b + 1u) * cap) return s;
+    b = b + 1u == nbins ? 0u : b + 1u;
+  }
+  return 0xFFFFFFFFu;
+}
+
+template <int kScatterThreads, int KIND, bool CAP>
 __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt)) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, PartFn pf, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
                                                                   uint32_t* __restrict__ keys_p,
-                                                                  uint32_t* __restrict__ pos_p) {
+                                                                  uint32_t* __restrict__ pos_p,
+                                                                  uint32_t bin_cap) {
   constexpr uint32_t kScatterTile = kScatterThreads * kScatterKpt;
   extern __shared__ uint32_t sm[];
   __shared__ uint32_t red[kScatterThreads / 64];
   uint32_t* hist = sm;
   uint32_t* lbase = sm + nparts;
   uint32_t* gbase = sm + 2 * nparts;
-  uint32_t* buf = sm + 3 * nparts;
+  uint32_t* fit = sm + 3 * nparts;  // CAP: keys of the tile's run that fit in the bin
+  uint32_t* buf = sm + (CAP ? 4 : 3) * nparts;
   const uint32_t tid = threadIdx.x;
   const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kScatterTile;
   const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
@@ -261,7 +279,15 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
   for (uint32_t b = b0; b < b1; ++b) {
     const uint32_t c = hist[b];
     lbase[b] = run;
-    if (c) gbase[b] = atomicAdd(&cursor[b * kLanes + (blockIdx.x & (kLanes - 1u))], c);
+    if (c) {
+      const uint32_t bin = b * kLanes + (blockIdx.x & (kLanes - 1u));
+      const uint32_t g = atomicAdd(&cursor[bin], c);
+      gbase[b] = g;
+      if constexpr (CAP) {
+        const uint32_t end = (bin + 1u) * bin_cap;
+        fit[b] = g >= end ? 0u : min(c, end - g);
+      }
+    }
     run += c;
   }
   __syncthreads();
@@ -281,8 +307,12 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
     if (sl < cnt) {
       const uint32_t k = buf[sl];
       const uint32_t p = part_of<KIND>(k, pf);
-      dst[r] = gbase[p] + (sl - lbase[p]);
-      keys_p[dst[r]] = k;
+      const uint32_t o = sl - lbase[p];
+      dst[r] = gbase[p] + o;
+      if constexpr (CAP) {
+        if (o >= fit[p]) dst[r] = spill_slot(cursor, nparts * kLanes, bin_cap, k, static_cast<uint32_t>(tile_base) + sl);
+      }
+      if (!CAP || dst[r] != 0xFFFFFFFFu) keys_p[dst[r]] = k;
     }
   }
   __syncthreads();
@@ -295,8 +325,14 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
-    if (sl < cnt) pos_p[dst[r]] = buf[sl];
+    if (sl < cnt && (!CAP || dst[r] != 0xFFFFFFFFu)) pos_p[dst[r]] = buf[sl];
   }
+}
+
+// Capacity bins: cursor[b] = b * cap, the start of bin b.
+__global__ void k_part_init_bins(uint32_t* __restrict__ cursor, uint32_t nbins, uint32_t cap) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbins) cursor[b] = b * cap;
 }
 
 // Hit sort: the matches' original positions hp[0 .. *d_hits) (any order)
@@ -458,17 +494,19 @@ static uint32_t fill_grid(uint64_t count) {
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((count + 255) / 256, 8192)));
 }
 
-static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile) { return nparts * 12u + tile * 4u; }
+static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile, bool cap) {
+  return nparts * (cap ? 16u : 12u) + tile * 4u;
+}
 
-template <int BT, int KIND>
+template <int BT, int KIND, bool CAP>
 static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
   constexpr uint32_t tile = BT * kScatterKpt;
-  const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile);
-  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KIND>), lds);
+  const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile, CAP);
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KIND, CAP>), lds);
   if (e != hipSuccess) return e;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
-  hipLaunchKernelGGL((k_part_scatter<BT, KIND>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf, L.nparts,
-                     L.cursor, L.keys_p, L.pos_p);
+  hipLaunchKernelGGL((k_part_scatter<BT, KIND, CAP>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf,
+                     L.nparts, L.cursor, L.keys_p, L.pos_p, L.bin_cap);
   return hipGetLastError();
 }
 
@@ -497,9 +535,9 @@ static hipError_t scan_and_scatter(const BbfPartLaunch& L, const PartFn& pf, con
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, L.stream, L.counts, L.nparts * kLanes,
                      L.offsets, L.cursor);
   switch (S.threads) {
-    case 1024: return launch_scatter<1024, KIND>(L, pf);
-    case 512: return launch_scatter<512, KIND>(L, pf);
-    default: return launch_scatter<256, KIND>(L, pf);
+    case 1024: return launch_scatter<1024, KIND, false>(L, pf);
+    case 512: return launch_scatter<512, KIND, false>(L, pf);
+    default: return launch_scatter<256, KIND, false>(L, pf);
   }
 }
 
@@ -527,12 +565,23 @@ static hipError_t partition_keys_kind(const BbfPartLaunch& L, const PartFn& pf) 
 
 hipError_t partition_keys(const BbfPartLaunch& L) {
   const PartFn pf = make_part_fn(L);
+  if (L.bin_cap) {  // capacity bins (blocked Bloom filters): no histogram pass
+    if (L.part_kind != 0) return hipErrorInvalidValue;
+    const uint32_t nbins = L.nparts * kLanes;
+    hipLaunchKernelGGL(k_part_init_bins, dim3((nbins + 255) / 256), dim3(256), 0, L.stream, L.cursor, nbins,
+                       L.bin_cap);
+    switch (part_shape(L).threads) {
+      case 1024: return launch_scatter<1024, 0, true>(L, pf);
+      case 512: return launch_scatter<512, 0, true>(L, pf);
+      default: return launch_scatter<256, 0, true>(L, pf);
+    }
+  }
   return L.part_kind == 1 ? partition_keys_kind<1>(L, pf) : partition_keys_kind<0>(L, pf);
 }
 
 hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint64_t* d_base, uint64_t* d_count) {
-  hipError_t e = scan_tile_counts(L.ptile_cnt, L.ntiles, d_count, L.stream);
-  if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ntiles, L.pos_p, hp, d_base, L.stream);
+  hipError_t e = scan_tile_counts(L.ptile_cnt, L.ptiles, d_count, L.stream);
+  if (e == hipSuccess) e = expand_gather(L.pbits, L.ptile_cnt, L.ptiles, L.pos_p, hp, d_base, L.stream);
   return e;
 }
 

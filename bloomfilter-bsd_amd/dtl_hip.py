@@ -18,6 +18,7 @@ PROBE_PARTITIONED = 2
 PROBE_DIRECT = 4
 PROBE_LANE_SPLIT = 8
 PROBE_ONE_LANE = 16
+PROBE_PART_HISTOGRAM = 32
 
 
 class BbfDesc(C.Structure):

@@ -135,7 +135,9 @@ enum {
   DTL_PROBE_DIRECT = 4u,         /* never use the partitioned probe */
   DTL_PROBE_LANE_SPLIT = 8u,     /* always read blocks lane-split where the configuration allows
                                     (and the filter is under 4 GiB) */
-  DTL_PROBE_ONE_LANE = 16u       /* never: one lane loads a key's whole block */
+  DTL_PROBE_ONE_LANE = 16u,      /* never: one lane loads a key's whole block */
+  DTL_PROBE_PART_HISTOGRAM = 32u /* partitioned probe of a blocked Bloom filter: size the partitions with
+                                    a histogram pass over the keys instead of fixed-capacity bins */
 };
 
 /* ---- descriptors -------------------------------------------------------- */

@@ -254,6 +254,50 @@ def test_partitioned_probe_matches_oracle(cfg, slice_log2, n_probe):
     assert np.array_equal(a, b)
 
 
+BIN_CASES = [
+    ((64, 1 << 25, 8, 8, 8, 1), 19, 1 << 20),        # 4 MiB, 512 KiB slices: 64 bins of 8 tiles
+    ((32, 1 << 25, 4, 1, 1, 1), 18, (1 << 20) + 777),  # register-blocked u32 w1, ragged last tile
+    ((32, 30000011, 6, 16, 16, 2), 18, 1200001),     # zoned MAGIC, lane-split spanning zones
+]
+
+
+@pytest.mark.parametrize("cfg,slice_log2,n_probe", BIN_CASES)
+def test_partitioned_capacity_bins_match_histogram_and_oracle(cfg, slice_log2, n_probe):
+    """Partitioned probe with fixed-capacity bins (no histogram pass; taken
+    when the bins hold 4 tiles or more) equals the histogram-sized form, the
+    direct kernel and the oracle."""
+    a = check_cfg(cfg, 60000, n_probe, seed=61, hit=0.1, offset=5, params=part_params(slice_log2))
+    p = part_params(slice_log2)
+    p.flags |= D.PROBE_PART_HISTOGRAM
+    b = check_cfg(cfg, 60000, n_probe, seed=61, hit=0.1, offset=5, params=p)
+    c = check_cfg(cfg, 60000, n_probe, seed=61, hit=0.1, offset=5,
+                  params=D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_DIRECT))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("dup_share", [0.3, 0.9])
+def test_partitioned_capacity_bins_spill_exact(dup_share):
+    """A batch in which most keys are one value (all in one partition) fills
+    that partition's bins far past their capacity: the keys that do not fit
+    take slots of other bins, and the list still equals the oracle's."""
+    cfg = (64, 1 << 25, 8, 8, 8, 1)
+    f = O.Bloom(*cfg)
+    desc = D.bbf_desc(*cfg)
+    rng = np.random.default_rng(62)
+    build = rng.integers(0, 2**31, 60000, dtype=np.uint64).astype(np.uint32)
+    words = gpu_build(desc, build)
+    ow = oracle_words(f, build)
+    n = 1 << 21
+    probe = mixed_keys(build, n, 0.05, 63)
+    idx = rng.choice(n, int(dup_share * n), replace=False)
+    probe[idx[: len(idx) // 2]] = build[7]            # a member: every copy matches
+    probe[idx[len(idx) // 2:]] = np.uint32(2**31 + 5)  # a non-member
+    exp = f.batch_contains(ow, probe, 11)
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 11, params=part_params(19))
+    torch.cuda.synchronize()
+    assert np.array_equal(D.positions(out, cnt), exp)
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 2047, 2049, 4095, 4096, 4097, 10000])
 def test_partitioned_probe_small_batches(n):
     cfg = (64, 1 << 20, 8, 8, 8, 1)
