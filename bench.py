@@ -719,7 +719,8 @@ def summarize_traffic(fetch_csv, write_csv, commit):
     then WRITE_SIZE, KiB per dispatch) of `bench.py --headline-only --calib-copy`:
     each counter is scaled by true bytes / counter bytes of the 1 GiB device copy
     that --calib-copy runs first; the phase is the last partitioned call
-    (k_fill_zero_u32 .. k_hit_bits), the direct kernel its last launch."""
+    (k_part_init_bins or, with a histogram pass, k_fill_zero_u32 .. k_hit_bits),
+    the direct kernel its last launch."""
     import csv
 
     def rows(path, counter):
@@ -740,9 +741,9 @@ def summarize_traffic(fetch_csv, write_csv, commit):
         name = name.split("(")[0].replace("void ", "").replace("dtlh::", "")
         return name.replace("unsigned long", "u64")
 
-    hist = [i for i, x in enumerate(f) if "k_part_hist<" in x[1]][-1]
-    first = max(i for i in range(hist) if "k_fill_zero_u32" in f[i][1])
-    last = next(i for i in range(hist, len(f)) if "k_hit_bits" in f[i][1])
+    scat = [i for i, x in enumerate(f) if "k_part_scatter<" in x[1]][-1]
+    first = max(i for i in range(scat) if "k_fill_zero_u32" in f[i][1] or "k_part_init_bins" in f[i][1])
+    last = next(i for i in range(scat, len(f)) if "k_hit_bits" in f[i][1])
     kern = [{"kernel": short(f[i][1]), "fetch_bytes": round(f[i][2] * fs), "write_bytes": round(w[i][2] * ws)}
             for i in range(first, last + 1)]
     fb, wb = sum(k["fetch_bytes"] for k in kern), sum(k["write_bytes"] for k in kern)

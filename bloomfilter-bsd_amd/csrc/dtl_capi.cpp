@@ -683,15 +683,27 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
+    // fixed-capacity bins as for Bloom filters (no histogram pass), where bins
+    // hold 4 tiles or more; DTL_PROBE_PART_HISTOGRAM keeps the histograms
+    uint32_t bin_cap = 0;
+    if (!(flags & DTL_PROBE_PART_HISTOGRAM) && n < (1ull << 31) && n >= nbins * 4ull * kTileKeys) {
+      const uint64_t want = (n + nbins - 1) / nbins * 103 / 100 + 256;
+      bin_cap = static_cast<uint32_t>((want + kTileKeys - 1) / kTileKeys * kTileKeys);
+    }
+    const uint32_t ptiles = bin_cap ? static_cast<uint32_t>(nbins * bin_cap / kTileKeys) : ntiles;
+    const size_t slots = static_cast<size_t>(ptiles) * kTileKeys;
+    const size_t pbits_bytes = static_cast<size_t>(ptiles) * kTileWords * 8;
+    const size_t pcnt_r = up((static_cast<size_t>(ptiles) + kScanTmpWords) * 4);
     const size_t part_bytes = up((nbins * 4 + 1) * 4);  // counts of both passes, offsets, cursors
     const size_t bucket_bytes = up(8 + (static_cast<size_t>(nbuckets) * 3 + 1) * 4);
     const size_t matrix_bytes = up(static_cast<size_t>(hit_groups) * nbuckets * 4);
-    const size_t key_bytes = up(static_cast<size_t>(n) * 4);
+    const size_t key_bytes = up(slots * 4);
     // a key can match in both passes: match positions take up to 2n words
     const size_t hit_bytes = up(static_cast<size_t>(n) * 8);
+    const size_t ppos_bytes = std::max(hit_bytes, up(slots * 4));  // grouped positions, then bucketed matches
     void* scratch = nullptr;
-    rc = get_scratch(dev, st, 2 * (bits_bytes + cnt_r) + part_bytes + bucket_bytes + 256 + matrix_bytes +
-                                  key_bytes + 2 * hit_bytes, &scratch);
+    rc = get_scratch(dev, st, bits_bytes + cnt_r + pbits_bytes + pcnt_r + part_bytes + bucket_bytes + 256 +
+                                  matrix_bytes + key_bytes + ppos_bytes + hit_bytes, &scratch);
     if (rc != DTL_OK) return rc;
     char* sp = static_cast<char*>(scratch);
     BbfPartLaunch P{};
@@ -718,9 +730,9 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     P.tile_cnt = reinterpret_cast<uint32_t*>(sp);
     sp += cnt_r;
     P.pbits = reinterpret_cast<uint64_t*>(sp);
-    sp += bits_bytes;
+    sp += pbits_bytes;
     P.ptile_cnt = reinterpret_cast<uint32_t*>(sp);
-    sp += cnt_r;
+    sp += pcnt_r;
     uint32_t* counts1 = reinterpret_cast<uint32_t*>(sp);
     uint32_t* counts2 = counts1 + nbins;
     P.offsets = counts2 + nbins;
@@ -742,10 +754,11 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     P.keys_p = reinterpret_cast<uint32_t*>(sp);
     sp += key_bytes;
     P.pos_p = reinterpret_cast<uint32_t*>(sp);  // grouped positions, later the bucketed matches (<= 2n)
-    sp += hit_bytes;
+    sp += ppos_bytes;
     uint32_t* hp = reinterpret_cast<uint32_t*>(sp);  // matches of both passes (<= 2n)
     P.ntiles = ntiles;
-    P.ptiles = ntiles;  // partitions sized by histograms (bin_cap 0)
+    P.bin_cap = bin_cap;
+    P.ptiles = ptiles;
     P.split = 0;
     P.cu_count = di.cu_count;
     P.stream = st;

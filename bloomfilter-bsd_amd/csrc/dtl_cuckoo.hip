@@ -112,7 +112,9 @@ __global__ __launch_bounds__(kGlobalThreads) void k_cf_probe_part(const uint8_t*
                                                                   uint32_t n, CfArgs a, uint32_t which,
                                                                   uint64_t* __restrict__ bits,
                                                                   uint32_t* __restrict__ tile_cnt,
-                                                                  uint32_t tiles_per_xcd, uint32_t ntiles) {
+                                                                  uint32_t tiles_per_xcd, uint32_t ntiles,
+                                                                  const uint32_t* __restrict__ bin_fill,
+                                                                  uint32_t tiles_per_bin) {
   constexpr int BT = kGlobalThreads;
   constexpr int ROUNDS = kTileKeys / BT;
   constexpr int U = ROUNDS;  // every key of the tile in flight (one bucket load each)
@@ -120,6 +122,13 @@ __global__ __launch_bounds__(kGlobalThreads) void k_cf_probe_part(const uint8_t*
   __shared__ uint32_t red[BT / 64];
   const uint32_t tile = (blockIdx.x & 7u) * tiles_per_xcd + (blockIdx.x >> 3);
   if (tile >= ntiles) return;  // uniform per workgroup, before any barrier
+  if (bin_fill) {  // keys grouped in bins of fixed capacity: this tile's bin ends at its fill
+    n = bin_end_of(bin_fill, tile, tiles_per_bin);
+    if (n <= tile * kTileKeys) {  // nothing of the bin in this tile (uniform, before any barrier)
+      clear_tile(bits, tile_cnt, tile);
+      return;
+    }
+  }
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t* vp = reinterpret_cast<const uint32_t*>(buf + a.victim_offset);
   const uint32_t v_index = vp[0];
@@ -170,10 +179,11 @@ hipError_t launch_cf_probe_part(const BbfPartLaunch& L) {
   a.bucket_cnt = L.cf_bucket_cnt;
   a.pad = 0;
   a.victim_offset = L.cf_victim_offset;
-  const uint32_t per_xcd = (L.ntiles + 7u) / 8u;
+  const uint32_t per_xcd = (L.ptiles + 7u) / 8u;
+  const uint32_t* fill = L.bin_cap ? L.cursor : nullptr;
   hipLaunchKernelGGL((k_cf_probe_part<T, B>), dim3(per_xcd * 8u), dim3(kGlobalThreads), 0, L.stream,
                      static_cast<const uint8_t*>(L.words), L.keys_p, L.n, a, L.cf_which, L.pbits, L.ptile_cnt,
-                     per_xcd, L.ntiles);
+                     per_xcd, L.ptiles, fill, L.bin_cap / kTileKeys);
   return hipGetLastError();
 }
 

@@ -565,16 +565,19 @@ static hipError_t partition_keys_kind(const BbfPartLaunch& L, const PartFn& pf) 
 
 hipError_t partition_keys(const BbfPartLaunch& L) {
   const PartFn pf = make_part_fn(L);
-  if (L.bin_cap) {  // capacity bins (blocked Bloom filters): no histogram pass
-    if (L.part_kind != 0) return hipErrorInvalidValue;
+  if (L.bin_cap) {  // capacity bins: no histogram pass
     const uint32_t nbins = L.nparts * kLanes;
     hipLaunchKernelGGL(k_part_init_bins, dim3((nbins + 255) / 256), dim3(256), 0, L.stream, L.cursor, nbins,
                        L.bin_cap);
-    switch (part_shape(L).threads) {
-      case 1024: return launch_scatter<1024, 0, true>(L, pf);
-      case 512: return launch_scatter<512, 0, true>(L, pf);
-      default: return launch_scatter<256, 0, true>(L, pf);
+    const uint32_t thr = part_shape(L).threads;
+    if (L.part_kind == 1) {
+      return thr == 1024 ? launch_scatter<1024, 1, true>(L, pf)
+             : thr == 512 ? launch_scatter<512, 1, true>(L, pf)
+                          : launch_scatter<256, 1, true>(L, pf);
     }
+    return thr == 1024 ? launch_scatter<1024, 0, true>(L, pf)
+           : thr == 512 ? launch_scatter<512, 0, true>(L, pf)
+                        : launch_scatter<256, 0, true>(L, pf);
   }
   return L.part_kind == 1 ? partition_keys_kind<1>(L, pf) : partition_keys_kind<0>(L, pf);
 }

@@ -401,7 +401,9 @@ def test_cuckoo_partitioned_small_batches(n):
 def test_cuckoo_partitioned_large_batch():
     """2^25 probe keys against a 32 MiB table (default-sized slices), both
     passes and the hit sort at scale, against the oracle's threaded
-    restatement (pinned to the scalar oracle by tests/test_cpu_baseline.py)."""
+    restatement (pinned to the scalar oracle by tests/test_cpu_baseline.py);
+    the passes' fixed-capacity bins (the default at this size) and
+    histogram-sized partitions give the same list."""
     import cpu_baseline as CB
     m = (1 << 22) * 8 * 8
     d = D.cf_desc(m, 16, 4)
@@ -421,6 +423,10 @@ def test_cuckoo_partitioned_large_batch():
     _, _, lists = CB.cf_probe(o, buf, probe, min(16, CB.physical_cores()))
     exp = np.concatenate(lists).astype(np.uint64) + 3
     assert len(got) == len(exp) and np.array_equal(got, exp)
+    ph = D.ProbeParams(0, 0, 0, 0, D.PROBE_PARTITIONED | D.PROBE_PART_HISTOGRAM)
+    out, cnt = D.cf_probe(d, torch.from_numpy(buf).to(dev()), to_dev(probe), 3, params=ph)
+    torch.cuda.synchronize()
+    assert np.array_equal(D.positions(out, cnt).astype(np.uint64), exp)
 
 
 def test_cuckoo_partitioned_many_partitions():
