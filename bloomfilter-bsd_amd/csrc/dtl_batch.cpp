@@ -31,6 +31,11 @@ struct PlanGroup {
 struct dtl_batch_plan {
   int device = 0;
   std::vector<PlanGroup> groups;   // one per distinct ((word_bits, w, s, z, k), size class)
+  // groups 1.. run on side streams forked from the caller's stream (default
+  // priority), joined back before the run returns; created on the first run
+  std::vector<hipStream_t> side;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
 };
 
 namespace {
@@ -46,7 +51,29 @@ void group_free(PlanGroup& g) {
 void plan_free(dtl_batch_plan* p) {
   if (!p) return;
   for (PlanGroup& g : p->groups) group_free(g);
+  for (hipStream_t s : p->side)
+    if (s) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : p->join)
+    if (e) (void)hipEventDestroy(e);
+  if (p->fork) (void)hipEventDestroy(p->fork);
   delete p;
+}
+
+// Side streams and fork / join events for the plan's groups 1.. (once).
+hipError_t plan_streams(dtl_batch_plan* p) {
+  const size_t need = p->groups.size() > 1 ? p->groups.size() - 1 : 0;
+  if (p->side.size() == need && (need == 0 || p->fork)) return hipSuccess;
+  hipError_t e = hipSuccess;
+  if (!p->fork) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+  while (e == hipSuccess && p->side.size() < need) {
+    hipStream_t s = nullptr;
+    hipEvent_t j = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if (s) p->side.push_back(s);
+    if (j) p->join.push_back(j);
+  }
+  return e;
 }
 
 bool same_config(const dtl_bbf_desc& a, const dtl_bbf_desc& b) {
@@ -203,12 +230,26 @@ int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
   return DTL_OK;
 }
 
+// The groups are independent (each has its own tasks, bitmap, counts and
+// work counter), so groups 1.. run on side streams forked from `stream` by an
+// event and joined back into it: a group's persistent kernel takes the CUs
+// that another group's tail leaves idle, and a group's scan and expand
+// overlap the others' probing.  Captured into a hipGraph, the fork and join
+// become graph dependencies.
 int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
   if (!p) return DTL_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   (void)hipGetLastError();  // report only this call's failures
-  hipError_t e = hipSuccess;
-  for (size_t g = 0; g < p->groups.size() && e == hipSuccess; ++g) e = group_run(p->groups[g], st);
+  hipError_t e = plan_streams(p);
+  const size_t ng = p->groups.size();
+  if (e == hipSuccess && ng > 1) e = hipEventRecord(p->fork, st);
+  for (size_t g = 1; g < ng && e == hipSuccess; ++g) {
+    e = hipStreamWaitEvent(p->side[g - 1], p->fork, 0);
+    if (e == hipSuccess) e = group_run(p->groups[g], p->side[g - 1]);
+    if (e == hipSuccess) e = hipEventRecord(p->join[g - 1], p->side[g - 1]);
+  }
+  if (e == hipSuccess && ng > 0) e = group_run(p->groups[0], st);
+  for (size_t g = 1; g < ng && e == hipSuccess; ++g) e = hipStreamWaitEvent(st, p->join[g - 1], 0);
   if (e != hipSuccess) {
     set_last_error(e);
     return DTL_EHIP;

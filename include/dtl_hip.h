@@ -237,12 +237,17 @@ typedef struct dtl_bbf_task {
 } dtl_bbf_task;
 
 /* A batch plan owns the device-side task table and scratch for a fixed set
- * of tasks; running it enqueues, per configuration group, four kernels
- * (clear, persistent probe over the group's tasks, per-task scan, expand)
- * with no allocation or memset, so a run can be captured into a hipGraph and
- * replayed.  Tasks may mix configurations (word_bits, w, s, z, k): they are
- * grouped in order of first appearance.  Task buffers are referenced, not
- * copied: their contents may change between runs. */
+ * of tasks; running it enqueues, per group, four kernels (clear, persistent
+ * probe over the group's tasks, per-task scan, expand) with no allocation or
+ * memset, so a run can be captured into a hipGraph and replayed.  Tasks may
+ * mix configurations (word_bits, w, s, z, k) and filter sizes: they are
+ * grouped by configuration and size class in order of first appearance.
+ * The first group runs on `stream`, the others on the plan's own side
+ * streams (default priority, created on the first run), forked from
+ * `stream` by an event and joined back into it before the call returns, so
+ * the run is ordered on `stream` as a whole (under capture the fork and join
+ * are graph edges).  Task buffers are referenced, not copied: their contents
+ * may change between runs; runs of one plan must not overlap. */
 typedef struct dtl_batch_plan dtl_batch_plan;
 
 int bbf_batch_plan_create(const dtl_bbf_task* tasks, uint32_t n_tasks,
