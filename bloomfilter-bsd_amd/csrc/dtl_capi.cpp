@@ -270,18 +270,42 @@ static uint32_t block_bytes(const dtl_bbf_desc* d) { return d->w * d->word_bits 
 // Default choice of the partitioned probe (Bloom and cuckoo filters): the
 // batch must be large enough for the partition passes to pay for
 // themselves, and the smaller the filter, the larger the batch it takes.
-// Measured on MI355X (u64 w8 s8 k8 Bloom filter; 16-bit / 4-slot cuckoo
-// table at 95% load; 1% hits), partitioned vs direct G probes/s:
-//   Bloom  8 MiB: 2^26 108 / 110, 2^28 123 / 111;  16 MiB: 2^24 73 / 73,
-//          2^26 107 / 74;  32 MiB: 2^24 72 / 61;  64 MiB: 2^24 72 / 55;
-//   cuckoo 8 MiB: 2^28 59 / 58;  16 MiB: 2^24 38 / 36, 2^26 52 / 37;
-//          32 MiB: 2^24 37 / 31;  64 MiB: 2^24 37 / 28.
+// Measured on MI355X with fixed-capacity bins (u64 w8 s8 k8 Bloom filter;
+// 16-bit / 4-slot cuckoo table at 95% load; 1% hits), partitioned vs direct
+// G probes/s:
+//   Bloom  8 MiB: 2^24 82 / 94, 2^25 105 / 101;  16 MiB: 2^23 61 / 62,
+//          2^24 82 / 67;  32 MiB: 2^22 41 / 48, 2^23 62 / 53;
+//          64 MiB: 2^22 40 / 43, 2^23 61 / 50;
+//   cuckoo 8 MiB: 2^24 47 / 52, 2^25 55 / 54;  16 MiB: 2^22 26 / 32,
+//          2^23 38 / 34;  32 MiB: 2^22 25 / 27, 2^23 37 / 29;
+//          64 MiB: 2^22 24 / 25, 2^23 36 / 26.
 static bool partition_pays(uint64_t filter_bytes, uint64_t n, bool cuckoo) {
-  if (filter_bytes >= (64ull << 20)) return n >= (1ull << 22);
   if (filter_bytes >= (32ull << 20)) return n >= (1ull << 23);
-  if (filter_bytes >= (16ull << 20)) return n >= (1ull << 25);
-  if (filter_bytes >= (8ull << 20)) return !cuckoo && n >= (1ull << 27);
+  if (filter_bytes >= (16ull << 20)) return n >= (cuckoo ? (1ull << 23) : (1ull << 24));
+  if (filter_bytes >= (8ull << 20)) return n >= (1ull << 25);
   return false;
+}
+
+// Capacity of each (partition, lane) bin of the partitioned probe, or 0 to
+// size the partitions with a histogram pass.  A full slice of
+// `slice_units` blocks (buckets) out of `total_units` receives n *
+// slice_units / total_units keys of a uniform batch, an eighth of them per
+// lane; the capacity is that + 3% + 256 keys (a bin's count has a standard
+// deviation of sqrt(expected), 0.4% at the 2^28-key benchmark), in whole
+// tiles.  The last slice may be partial, so the bins of the full ones must
+// cover their own share, not n / bins.  Bins are used where they hold 4
+// tiles or more and the grouped order stays under 2^31 slots (the scatter
+// marks spilled keys with the top bit).
+static uint32_t bin_capacity(uint64_t n, uint32_t nparts, uint64_t slice_units, uint64_t total_units,
+                             uint32_t flags) {
+  if ((flags & DTL_PROBE_PART_HISTOGRAM) || n >= (1ull << 30) || total_units == 0) return 0;
+  const double share = static_cast<double>(n) * static_cast<double>(std::min(slice_units, total_units)) /
+                       static_cast<double>(total_units) / kPartLanes;
+  if (share < 4.0 * kTileKeys) return 0;
+  const uint64_t want = static_cast<uint64_t>(share * 1.03) + 257;
+  const uint64_t cap = (want + kTileKeys - 1) / kTileKeys * kTileKeys;
+  if (cap * nparts * kPartLanes >= (1ull << 31)) return 0;
+  return static_cast<uint32_t>(cap);
 }
 
 static bool aligned_to(const void* p, uint32_t a) {
@@ -485,17 +509,9 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
-    // Bins of fixed capacity (no histogram pass) where they hold 4 tiles of
-    // keys or more: the expected share + 3% + 256 keys (a bin's count has a
-    // standard deviation of sqrt(expected), 0.4% at the 2^28-key benchmark),
-    // whole tiles; keys past a full bin go to other bins.  Smaller batches,
-    // batches of 2^31 keys or more, and DTL_PROBE_PART_HISTOGRAM size the
-    // partitions with a histogram pass instead.
-    uint32_t bin_cap = 0;
-    if (!(flags & DTL_PROBE_PART_HISTOGRAM) && n < (1ull << 31) && n >= nbins * 4ull * kTileKeys) {
-      const uint64_t want = (n + nbins - 1) / nbins * 103 / 100 + 256;
-      bin_cap = static_cast<uint32_t>((want + kTileKeys - 1) / kTileKeys * kTileKeys);
-    }
+    // bins of fixed capacity (no histogram pass), see bin_capacity; keys past
+    // a full bin go to other bins
+    const uint32_t bin_cap = bin_capacity(n, nparts, 1ull << slice_shift, d->block_cnt, flags);
     const uint32_t ptiles = bin_cap ? static_cast<uint32_t>(nbins * bin_cap / kTileKeys) : ntiles;
     const size_t pbits_bytes = static_cast<size_t>(ptiles) * kTileWords * 8;
     const size_t pcnt_r = up((static_cast<size_t>(ptiles) + kScanTmpWords) * 4);
@@ -683,13 +699,9 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t cnt_r = up(cnt_bytes);
     const size_t nbins = static_cast<size_t>(nparts) * kPartLanes;
-    // fixed-capacity bins as for Bloom filters (no histogram pass), where bins
-    // hold 4 tiles or more; DTL_PROBE_PART_HISTOGRAM keeps the histograms
-    uint32_t bin_cap = 0;
-    if (!(flags & DTL_PROBE_PART_HISTOGRAM) && n < (1ull << 31) && n >= nbins * 4ull * kTileKeys) {
-      const uint64_t want = (n + nbins - 1) / nbins * 103 / 100 + 256;
-      bin_cap = static_cast<uint32_t>((want + kTileKeys - 1) / kTileKeys * kTileKeys);
-    }
+    // fixed-capacity bins as for Bloom filters (no histogram pass), see
+    // bin_capacity; DTL_PROBE_PART_HISTOGRAM keeps the histograms
+    const uint32_t bin_cap = bin_capacity(n, nparts, 1ull << slice_shift, d->bucket_cnt, flags);
     const uint32_t ptiles = bin_cap ? static_cast<uint32_t>(nbins * bin_cap / kTileKeys) : ntiles;
     const size_t slots = static_cast<size_t>(ptiles) * kTileKeys;
     const size_t pbits_bytes = static_cast<size_t>(ptiles) * kTileWords * 8;

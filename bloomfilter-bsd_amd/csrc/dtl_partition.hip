@@ -218,11 +218,20 @@ __global__ __launch_bounds__(kPartScanThreads) void k_part_scan(const uint32_t* 
 // workgroups fit on a CU: one loads and sorts while the other writes (the
 // launch bound asks for 2 workgroups' waves per SIMD, <= 64 VGPRs at 1024
 // threads).
-// Slot for a key that did not fit in its full bin (capacity bins): one slot
-// reserved at a time from bins probed from a hash-chosen start, so every
-// reserved slot below a bin's capacity is written.  The bins' capacities add
-// up to at least n, so a free slot exists; the bound on the walk only keeps
-// the loop finite.
+// Capacity bins: the keys of a tile that do not fit in their (full) bins --
+// none for uniform keys, many when a batch repeats a few keys -- are spilled
+// to other bins.  The workgroup reserves all of them at once, in up to
+// kSpillSegs chunks taken from bins walked from a start chosen by the
+// workgroup (one atomicAdd per chunk; a chunk may fit only partly, and only
+// the slots below the bin's capacity are used, all of them written); keys
+// beyond those chunks take one slot at a time (spill_slot).  The bins'
+// capacities add up to more than n, so free slots exist; the bounds on the
+// walks only keep the loops finite.  Where a key sits decides locality only:
+// the probe computes every key's block from the key.
+constexpr uint32_t kSpillSegs = 32;
+constexpr uint32_t kSpillMark = 0x80000000u;  // dst of a spilled key: mark | its index among the tile's spills
+
+// One slot for a spilled key beyond the workgroup's chunks.
 __device__ __forceinline__ uint32_t spill_slot(uint32_t* __restrict__ cursor, uint32_t nbins, uint32_t cap,
                                                uint32_t key, uint32_t pos) {
   uint32_t b = ((key ^ (pos * 0x9E3779B9u)) * 0x85EBCA6Bu) % nbins;
@@ -248,6 +257,8 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
   uint32_t* lbase = sm + nparts;
   uint32_t* gbase = sm + 2 * nparts;
   uint32_t* fit = sm + 3 * nparts;  // CAP: keys of the tile's run that fit in the bin
+  __shared__ uint32_t s_nspill, s_nseg, s_placed;
+  __shared__ uint32_t s_seg[kSpillSegs][3];  // CAP: (first spill index, grouped slot, count)
   uint32_t* buf = sm + (CAP ? 4 : 3) * nparts;
   const uint32_t tid = threadIdx.x;
   const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kScatterTile;
@@ -301,6 +312,10 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
   }
   __syncthreads();
   uint32_t dst[kScatterKpt];
+  if constexpr (CAP) {
+    if (tid == 0) s_nspill = 0u;
+    __syncthreads();
+  }
 #pragma unroll
   for (int r = 0; r < kScatterKpt; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
@@ -310,9 +325,54 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
       const uint32_t o = sl - lbase[p];
       dst[r] = gbase[p] + o;
       if constexpr (CAP) {
-        if (o >= fit[p]) dst[r] = spill_slot(cursor, nparts * kLanes, bin_cap, k, static_cast<uint32_t>(tile_base) + sl);
+        if (o >= fit[p]) dst[r] = kSpillMark | atomicAdd(&s_nspill, 1u);
       }
-      if (!CAP || dst[r] != 0xFFFFFFFFu) keys_p[dst[r]] = k;
+      if (!CAP || !(dst[r] & kSpillMark)) keys_p[dst[r]] = k;
+    }
+  }
+  if constexpr (CAP) {
+    __syncthreads();
+    const uint32_t nspill = s_nspill;  // uniform
+    if (nspill) {
+      if (tid == 0) {
+        const uint32_t nbins = nparts * kLanes;
+        uint32_t b = (blockIdx.x * 0x9E3779B1u) % nbins, done = 0, nseg = 0;
+        for (uint32_t t = 0; t < 2u * nbins && done < nspill && nseg < kSpillSegs; ++t) {
+          const uint32_t want = nspill - done;
+          const uint32_t g = atomicAdd(&cursor[b], want);
+          const uint32_t end = (b + 1u) * bin_cap;
+          const uint32_t f = g >= end ? 0u : min(want, end - g);
+          if (f) {
+            s_seg[nseg][0] = done;
+            s_seg[nseg][1] = g;
+            s_seg[nseg][2] = f;
+            ++nseg;
+            done += f;
+          }
+          b = b + 1u == nbins ? 0u : b + 1u;
+        }
+        s_nseg = nseg;
+        s_placed = done;
+      }
+      __syncthreads();
+      const uint32_t nseg = s_nseg, placed = s_placed;
+#pragma unroll
+      for (int r = 0; r < kScatterKpt; ++r) {
+        const uint32_t sl = r * kScatterThreads + tid;
+        if (sl < cnt && (dst[r] & kSpillMark)) {
+          const uint32_t idx = dst[r] & ~kSpillMark;
+          const uint32_t k = buf[sl];
+          uint32_t d = 0xFFFFFFFFu;
+          if (idx < placed) {
+            for (uint32_t q = 0; q < nseg; ++q)
+              if (idx >= s_seg[q][0] && idx < s_seg[q][0] + s_seg[q][2]) d = s_seg[q][1] + (idx - s_seg[q][0]);
+          } else {
+            d = spill_slot(cursor, nparts * kLanes, bin_cap, k, static_cast<uint32_t>(tile_base) + sl);
+          }
+          dst[r] = d;
+          if (d != 0xFFFFFFFFu) keys_p[d] = k;
+        }
+      }
     }
   }
   __syncthreads();

@@ -122,8 +122,8 @@ typedef struct dtl_probe_params {
 } dtl_probe_params;
 
 /* Probe strategy.  By default filters that fit in LDS are staged there;
- * filters of at least 64 MiB probed with at least 2^22 keys (32 MiB: 2^23
- * keys, 16 MiB: 2^25, 8 MiB: 2^27) use the partitioned probe (keys grouped by
+ * filters of at least 32 MiB probed with at least 2^23 keys (16 MiB: 2^24
+ * keys, 8 MiB: 2^25) use the partitioned probe (keys grouped by
  * filter slice so that each slice is read into L2 once); everything else
  * reads blocks straight from memory.  Blocks
  * of 32 B or more whose bits stay inside 16-byte pieces (s >= w; sectors or
@@ -184,8 +184,8 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
 int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,
              uint32_t match_offset, uint32_t* d_pos, uint64_t* d_count, dtl_stream_t stream);
 /* cf_probe with explicit parameters (NULL = defaults; same list either way).
- * Tables of 64 MiB or more probed with at least 2^22 keys (32 MiB: 2^23
- * keys, 16 MiB: 2^25) use the partitioned probe: keys grouped by the slice of their first bucket, that
+ * Tables of 16 MiB or more probed with at least 2^23 keys (8 MiB: 2^25)
+ * use the partitioned probe: keys grouped by the slice of their first bucket, that
  * bucket and the victim cache tested from L2, then regrouped by the slice of
  * their alternate bucket and that bucket tested; ~20 bytes of scratch per key.
  * Honoured: flags DTL_PROBE_DIRECT / DTL_PROBE_PARTITIONED, slice_log2,

@@ -684,18 +684,19 @@ def test_large_batches_exact(ntiles, path):
     assert np.array_equal(got.astype(np.uint64), exp)
 
 
-@pytest.mark.parametrize("log2_m,log2_n", [(27, 25), (28, 23), (26, 27)])
+@pytest.mark.parametrize("log2_m,log2_n", [(27, 24), (28, 23), (26, 25)])
 def test_default_path_at_break_even_sizes(log2_m, log2_n):
-    """16 MiB with 2^25 keys, 32 MiB with 2^23 and 8 MiB with 2^27 take the
-    partitioned probe by default: the default call's list equals the forced
-    direct and forced partitioned ones and the oracle's on every key."""
+    """16 MiB with 2^24 + 3 keys, 32 MiB with 2^23 + 3 and 8 MiB with 2^25 + 3
+    take the partitioned probe by default (fixed-capacity bins): the default
+    call's list equals the forced direct and forced partitioned ones and the
+    oracle's on every key."""
     cfg = (64, 1 << log2_m, 8, 8, 8, 1)
     f = O.Bloom(*cfg)
     desc = D.bbf_desc(*cfg)
     build = np.random.default_rng(log2_m).integers(0, 2**31, (1 << log2_m) // 12, dtype=np.uint64).astype(np.uint32)
     words = gpu_build(desc, build)
     ow = oracle_words(f, build)
-    probe = big_probe_keys(build, (1 << log2_n) - 5, log2_n)
+    probe = big_probe_keys(build, (1 << log2_n) + 3, log2_n)
     kd = to_dev(probe)
     lists = []
     for flags in (0, D.PROBE_DIRECT, D.PROBE_PARTITIONED):
