@@ -12,7 +12,8 @@ element-wise comparison of its list with the CPU baseline's (outside the
 timed region):
   regimes            the same filter at 16 KiB, 4 MiB and 128 MiB (2^28 keys), and
                      the plain register-blocked filter (u32, w=1, k=4, 128 KiB,
-                     2^16 keys; 2^24 probes at 10% hits);
+                     2^16 keys; 2^24 probes at 10% hits), the latter also as 16
+                     calls captured in one hipGraph;
   cuckoo             16-bit tags, 4 slots, 2^26 buckets at 95% load (host build),
                      2^28 probes at 1% and 50% hits;
   learned_config     the calibration sweep, the MLP and its regret against the
@@ -196,7 +197,8 @@ GATHER_MODES = {0: "1 lane / 64 B sector, 4x16 B loads", 1: "as 0, non-temporal"
                 2: "4 lanes / 64 B sector, 1x16 B each", 3: "as 2, non-temporal",
                 4: "1 lane / 32 B sector, 2x16 B loads", 5: "2 lanes / 32 B sector",
                 6: "as 2, uniform random sectors", 7: "as 0, uniform random sectors",
-                8: "LDS-staged buffer, 1 lane / pseudo-random 4 B word (per-lane LCG)"}
+                8: "LDS-staged buffer, 1 lane / pseudo-random 4 B word (per-lane LCG)",
+                9: "LDS-staged buffer, 1 lane / pseudo-random 64 B block, 4x16 B reads in the probe's rotated layout"}
 
 
 def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
@@ -224,10 +226,42 @@ def gather_table(torch, D, modes, n=1 << 28, reps=5, log2_bytes=30):
     return out
 
 
+def graph_replay_ms(torch, D, desc, words, probe, calls, reps, expect):
+    """Per-call time of `calls` back-to-back bbf_probe calls captured in one
+    hipGraph and replayed `reps` times (the caller's launch gaps between calls
+    go away; the kernels are the same).  The list of the last call is checked
+    against `expect` after the replays."""
+    n = probe.numel()
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        D.bbf_probe(desc, words, probe, 0, out=out, count=cnt)   # sizes the stream's scratch before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(calls):
+            D.bbf_probe(desc, words, probe, 0, out=out, count=cnt)
+    g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    c = int(cnt.item())
+    same = c == expect.numel() and bool(torch.equal(out[:c], expect))
+    del g
+    return a.elapsed_time(b) / (reps * calls), same
+
+
 def regime(torch, D, CB, O, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, steps, warmup, label,
-           n_build=None, no_cpu=False):
+           n_build=None, no_cpu=False, graph_calls=0):
     """Whole-job probe rate of one filter configuration and size, with the
-    CPU baseline on the same batch and their lists compared."""
+    CPU baseline on the same batch and their lists compared.  With
+    graph_calls, also the rate of that many calls captured in one hipGraph."""
     import numpy as np
     desc = D.bbf_desc(word_bits, 1 << log2_m, k, w, s, z)
     build, probe, n_build = make_inputs(torch, log2_m, n_probe, hit_rate, 7, n_build=n_build)
@@ -236,6 +270,11 @@ def regime(torch, D, CB, O, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, st
     torch.cuda.synchronize()
     st = {}
     ms, kern_ms, matches, pos = time_probe(torch, D, desc, words, probe, steps, warmup, per_step=st, warm_ms=200.0)
+    graph = None
+    if graph_calls:
+        g_ms, g_same = graph_replay_ms(torch, D, desc, words, probe, graph_calls, 20, pos)
+        graph = {"calls_per_graph": graph_calls, "replays": 20, "ms_per_call": round(g_ms, 4),
+                 "G_probes_per_s": round(n_probe / (g_ms * 1e-3) / 1e9, 2), "list_identical": g_same}
     bpk = 4.0 + 4.0 * matches / n_probe   # key + match from HBM; the filter is read once per workgroup or from cache
     res = {"config": label, "filter_bytes": int(desc.size_bytes),
            "path": "lds" if desc.size_bytes <= 158 * 1024 else "global",
@@ -243,6 +282,8 @@ def regime(torch, D, CB, O, word_bits, log2_m, k, w, s, z, n_probe, hit_rate, st
            "kernel_G_probes_per_s": round(n_probe / (kern_ms * 1e-3) / 1e9, 2),
            "call_ms": st, "matches": matches,
            "hbm_bytes_per_probe": round(bpk + (0.0 if desc.size_bytes <= 158 * 1024 else 64.0), 3)}
+    if graph is not None:
+        res["graph_replay"] = graph
     if not no_cpu:
         f = O.Bloom(word_bits, 1 << log2_m, k, w, s, z)
         gpu = pos.cpu().numpy().view(np.uint32).astype(np.uint64)
@@ -269,14 +310,22 @@ def run_regimes(torch, D, CB, O, steps_arg, no_cpu=False):
                    no_cpu=no_cpu),
             regime(torch, D, CB, O, 32, 20, 4, 1, 1, 1, 1 << 24, 0.10, steps, warm,
                    "plain register-blocked u32 w1 s1 k4, 128 KiB (2^16 keys), 2^24 probes at 10% hits",
-                   n_build=1 << 16, no_cpu=no_cpu)]
-    for r, lb in ((regs[0], 14), (regs[3], 17)):   # the LDS-staged regimes: 16 KiB and 128 KiB filters
-        lds = gather_table(torch, D, [8], reps=3, log2_bytes=lb)[8]
+                   n_build=1 << 16, no_cpu=no_cpu, graph_calls=16)]
+    # the LDS-staged regimes against the measured LDS read rate of their own
+    # access: one random 64 B block per probe for the 16 KiB u64 w8 filter
+    # (mode 9), one random 4-byte word for the 128 KiB u32 w1 filter (mode 8)
+    for r, lb, mode, what in ((regs[0], 14, 9, "64 B block"), (regs[3], 17, 8, "4-byte word")):
+        lds = gather_table(torch, D, [mode], reps=3, log2_bytes=lb)[mode]
         r["roofline"] = {"bound": "lds", "achieved": round(r["kernel_G_probes_per_s"], 2),
-                         "peak": lds["G_sectors_per_s"], "unit": "G random 4-byte LDS reads/s",
+                         "peak": lds["G_sectors_per_s"], "unit": "G random %s LDS reads/s" % what,
                          "frac": round(r["kernel_G_probes_per_s"] / lds["G_sectors_per_s"], 3),
-                         "note": "one LDS word read per probe at least; the LDS rate is measured "
-                                 "(dtl_gather_bench mode 8, buffer of %d bytes)" % (1 << lb)}
+                         "lds_GB_per_s": lds["useful_GB_per_s"],
+                         "note": "one random LDS %s read per probe; the LDS rate is measured "
+                                 "(dtl_gather_bench mode %d, buffer of %d bytes); the key stream "
+                                 "from HBM (4 B per probe) comes on top" % (what, mode, 1 << lb)}
+        hbm = r["kernel_G_probes_per_s"] * r["hbm_bytes_per_probe"]
+        r["roofline"]["key_stream_hbm"] = {"achieved_GB_per_s": round(hbm, 1), "peak": HBM_PEAK_GBS,
+                                           "frac": round(hbm / HBM_PEAK_GBS, 4)}
     for r in regs[1:3]:
         bpk = 4.0 + 64.0 + 4.0 * r["matches"] / (1 << 28)
         ach = bpk * r["kernel_G_probes_per_s"]
@@ -842,6 +891,8 @@ def main():
         for lb in (21, 22, 24, 27, 30):
             res["by_size_%dMiB" % (1 << (lb - 20))] = {
                 m: r["G_sectors_per_s"] for m, r in gather_table(torch, D, (0, 2, 6, 7), reps=3, log2_bytes=lb).items()}
+        for lb in (14, 17):
+            res["lds_%dKiB" % (1 << (lb - 10))] = gather_table(torch, D, (8, 9), reps=3, log2_bytes=lb)
         print(json.dumps(res), flush=True)
         return
 
@@ -908,9 +959,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
             "traffic_measured_at": traffic.get("commit") if traffic else None,
-            "kernel": ("partitioned probe phase (k_part_hist, k_part_scatter, k_bbf_probe_split over the "
-                       "grouped keys: 4 lanes per key, one 16 B piece of the 64 B block each; match "
-                       "reorder), HIP events around the phase" if partitioned
+            "kernel": ("partitioned probe phase (k_part_init_bins, k_part_scatter into fixed-capacity "
+                       "bins, k_bbf_probe_split over the grouped keys: 4 lanes per key, one 16 B piece "
+                       "of the 64 B block each; match reorder), HIP events around the phase" if partitioned
                        else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
             "kernel_ms_per_call": per_call,

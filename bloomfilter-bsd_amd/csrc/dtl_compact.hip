@@ -1,7 +1,8 @@
 // dtl_compact.hip -- order-preserving compaction of the match bitmap.
 //
 // Phase 2 (scan): one workgroup per task scans the task's tile counts in
-// place into exclusive offsets and writes the task's match count.
+// place into exclusive offsets and writes the task's match count (single
+// lists of up to 8192 tiles skip it: their expand adds up the counts itself).
 // Phase 3 (expand): one workgroup per group of up to 8 tiles; 256 threads
 // each own 8 keys (one byte of the bitmap) of every tile of the group, one
 // workgroup exclusive scan of the byte popcounts gives every match its rank,
@@ -133,13 +134,17 @@ static_assert(kExpandThreads * 8 == kTileKeys, "one bitmap byte per thread");
 // per-tile byte popcounts packed 16 bits each into two 64-bit values; a tile
 // has at most 2048 matches, so fields never carry into each other), and
 // emit(tile, key index inside the tile, output slot) runs for every match in
-// key order within its tile.
+// key order within its tile.  off_of(r) is the first output slot of tile
+// t0 + r; it is read before the scans, so its latency overlaps theirs.
 constexpr int kExpandGroup = 8;
 
-template <class Emit>
+template <class OffOf, class Emit>
 __device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
-                                             const uint32_t* __restrict__ offs, uint64_t* sh, Emit&& emit) {
+                                             OffOf&& off_of, uint64_t* sh, Emit&& emit) {
   const uint32_t t = threadIdx.x;
+  uint32_t base[kExpandGroup];
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) base[r] = off_of(static_cast<uint32_t>(r) < nt ? r : nt - 1u);
   uint32_t b8[kExpandGroup];
   // Every tile slot loads (tiles past the group's end re-read its last tile
   // and are masked): straight-line loads, no branches around them.  A form
@@ -167,7 +172,7 @@ __device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, 
   for (int r = 0; r < kExpandGroup; ++r) {
     if (static_cast<uint32_t>(r) >= nt) break;
     const uint64_t ex = r < 4 ? elo : ehi;
-    uint32_t o = offs[t0 + r] + static_cast<uint32_t>((ex >> (16 * (r % 4))) & 0xFFFFu);
+    uint32_t o = base[r] + static_cast<uint32_t>((ex >> (16 * (r % 4))) & 0xFFFFu);
     uint32_t b = b8[r];
     while (b) {
       const uint32_t j = __ffs(b) - 1u;
@@ -194,9 +199,9 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t
   __shared__ uint64_t sh[kExpandThreads / 64];
   const uint32_t t0 = blockIdx.x * per_wg;
   uint32_t* dst = out + (d_base ? *d_base : 0ull);
-  expand_group(bits, t0, min(per_wg, ntiles - t0), offs, sh, [&](uint32_t tile, uint32_t i, uint32_t o) {
-    dst[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i];
-  });
+  expand_group(
+      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh,
+      [&](uint32_t tile, uint32_t i, uint32_t o) { dst[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i]; });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
@@ -206,9 +211,46 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t
                                                                   uint32_t* __restrict__ pos) {
   __shared__ uint64_t sh[kExpandThreads / 64];
   const uint32_t t0 = blockIdx.x * per_wg;
-  expand_group(bits, t0, min(per_wg, ntiles - t0), offs, sh, [&](uint32_t tile, uint32_t i, uint32_t o) {
-    pos[o] = tile * kTileKeys + i + match_offset;
-  });
+  expand_group(
+      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh,
+      [&](uint32_t tile, uint32_t i, uint32_t o) { pos[o] = tile * kTileKeys + i + match_offset; });
+}
+
+// Short lists (at most kFusedTiles tiles): the scan folded into the expand.
+// Workgroup g adds up the tile counts before its group itself (at most
+// kFusedTiles counts, read from L2), so no scan launch runs between the probe
+// and the expand; the workgroup holding the last tile writes the total.  The
+// counts are read, not overwritten.
+constexpr uint32_t kFusedTiles = 8192;
+
+__global__ __launch_bounds__(kExpandThreads) void k_expand_fused(const uint64_t* __restrict__ bits,
+                                                                 const uint32_t* __restrict__ cnt,
+                                                                 uint32_t ntiles, uint32_t per_wg,
+                                                                 uint32_t match_offset,
+                                                                 uint32_t* __restrict__ pos,
+                                                                 uint64_t* __restrict__ d_count) {
+  __shared__ uint64_t sh[kExpandThreads / 64];
+  __shared__ uint32_t sh32[kExpandThreads / 64];
+  __shared__ uint32_t s_off[kExpandGroup];
+  const uint32_t t0 = blockIdx.x * per_wg;
+  const uint32_t nt = min(per_wg, ntiles - t0);
+  uint32_t part = 0;
+#pragma unroll 8
+  for (uint32_t j = threadIdx.x; j < t0; j += kExpandThreads) part += cnt[j];
+  uint32_t before;
+  (void)block_excl_scan<kExpandThreads>(part, before, sh32);  // before = sum of cnt[0 .. t0)
+  if (threadIdx.x == 0) {
+    uint32_t o = before;
+    for (uint32_t r = 0; r < nt; ++r) {
+      s_off[r] = o;
+      o += cnt[t0 + r];
+    }
+    if (t0 + nt == ntiles) *d_count = o;
+  }
+  __syncthreads();
+  expand_group(
+      bits, t0, nt, [&](uint32_t r) { return s_off[r]; }, sh,
+      [&](uint32_t tile, uint32_t i, uint32_t o) { pos[o] = tile * kTileKeys + i + match_offset; });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
@@ -218,10 +260,12 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t*
                                                                  uint32_t total_tiles, uint32_t per_wg) {
   __shared__ uint64_t sh[kExpandThreads / 64];
   const uint32_t g0 = blockIdx.x * per_wg;
-  expand_group(bits, g0, min(per_wg, total_tiles - g0), offs, sh, [&](uint32_t g, uint32_t i, uint32_t o) {
-    const BatchTaskDev& t = tasks[tile_task[g]];
-    t.pos[o] = (g - t.tile_begin) * kTileKeys + i + t.match_offset;
-  });
+  expand_group(
+      bits, g0, min(per_wg, total_tiles - g0), [&](uint32_t r) { return offs[g0 + r]; }, sh,
+      [&](uint32_t g, uint32_t i, uint32_t o) {
+        const BatchTaskDev& t = tasks[tile_task[g]];
+        t.pos[o] = (g - t.tile_begin) * kTileKeys + i + t.match_offset;
+      });
 }
 
 __global__ void k_zero_count(uint64_t* d_count) { *d_count = 0; }
@@ -232,6 +276,14 @@ hipError_t compact_single(const uint64_t* bits, uint32_t* tile_cnt, uint32_t nti
                           uint32_t match_offset, uint32_t* pos, uint64_t* d_count,
                           hipStream_t stream) {
   (void)n;
+  if (ntiles <= kFusedTiles) {
+    // fewer tiles per workgroup for shorter lists; the prefix sums then read
+    // at most ntiles^2 / (2 per) counts in all
+    const uint32_t per = ntiles >= 4096 ? 8u : ntiles >= 1024 ? 4u : ntiles >= 256 ? 2u : 1u;
+    hipLaunchKernelGGL(k_expand_fused, dim3((ntiles + per - 1) / per), dim3(kExpandThreads), 0, stream, bits,
+                       tile_cnt, ntiles, per, match_offset, pos, d_count);
+    return hipGetLastError();
+  }
   hipError_t e = scan_counts(tile_cnt, ntiles, d_count, stream);
   if (e != hipSuccess) return e;
   const uint32_t per = expand_tiles(ntiles);

@@ -15,6 +15,9 @@
 //   8  LDS: every workgroup stages the whole buffer (at most 128 KiB) in LDS,
 //      then each lane reads pseudo-random 4-byte words of it (the access of
 //      a one-word-block filter probed from LDS)
+//   9  LDS as 8, but each lane reads a pseudo-random 64 B block as four
+//      16-byte pieces, piece p of block b from slot 4 b + ((p + b / 4) mod 4)
+//      (the rotated layout in which the probe stages 64-byte blocks)
 // Modes 0-5 pick sector (i * 0x9E3779B1) >> shift, a low-discrepancy
 // sequence that spreads consecutive reads evenly over the memory channels;
 // modes 6-7 pick independent uniform sectors, like the blocks of random keys.
@@ -71,31 +74,49 @@ __global__ __launch_bounds__(256) void k_gather(const u32x4* __restrict__ buf, u
   if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9E3779B1u) sink[0] = gid;  // keeps the loads alive
 }
 
-constexpr int kLdsReadsPerIter = 8;
-
+// BLOCKS: 64 B blocks (mode 9) instead of 4-byte words (mode 8); `shift`
+// maps a 32-bit draw to a word or block index of the buffer.
+template <bool BLOCKS>
 __global__ __launch_bounds__(1024) void k_lds_gather(const u32x4* __restrict__ buf, uint32_t buf_bytes,
                                                      uint32_t shift, uint64_t n_reads,
                                                      uint64_t* __restrict__ sink) {
+  constexpr int R = BLOCKS ? 4 : 8;  // reads per lane per iteration
   extern __shared__ u32x4 lds_buf[];
   for (uint32_t i = threadIdx.x; i < buf_bytes / 16; i += blockDim.x) lds_buf[i] = buf[i];
   __syncthreads();
   const uint32_t* words = reinterpret_cast<const uint32_t*>(lds_buf);
   const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kLdsReadsPerIter;
-  // word indexes: the top bits of a per-lane linear congruential stream
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * R;
+  // indexes: the top bits of a per-lane linear congruential stream
   // (Numerical Recipes' 1664525 / 1013904223), seeded by a full-avalanche
   // hash of the lane, so that the index arithmetic costs two operations
   uint32_t x = sector_of_random(gid, 0);
   uint32_t acc = 0;
-  for (uint64_t base = gid * kLdsReadsPerIter; base < n_reads; base += stride) {
-    uint32_t v[kLdsReadsPerIter];
+  for (uint64_t base = gid * R; base < n_reads; base += stride) {
+    if constexpr (BLOCKS) {
+      u32x4 v[R][4];
 #pragma unroll
-    for (int r = 0; r < kLdsReadsPerIter; ++r) {
-      x = x * 1664525u + 1013904223u;
-      v[r] = words[x >> shift];
+      for (int r = 0; r < R; ++r) {
+        x = x * 1664525u + 1013904223u;
+        const uint32_t b = x >> shift;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[r][p] = lds_buf[b * 4u + ((static_cast<uint32_t>(p) + (b >> 2)) & 3u)];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const u32x4 t = v[r][0] ^ v[r][1] ^ v[r][2] ^ v[r][3];
+        acc ^= (base + r < n_reads) ? (t.x ^ t.y ^ t.z ^ t.w) : 0u;
+      }
+    } else {
+      uint32_t v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        x = x * 1664525u + 1013904223u;
+        v[r] = words[x >> shift];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc ^= (base + r < n_reads) ? v[r] : 0u;
     }
-#pragma unroll
-    for (int r = 0; r < kLdsReadsPerIter; ++r) acc ^= (base + r < n_reads) ? v[r] : 0u;
   }
   if (acc == 0x9E3779B1u) sink[0] = acc;  // keeps the reads alive
 }
@@ -117,9 +138,9 @@ extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t 
                                 uint64_t* d_sink, dtl_stream_t stream) {
   using namespace dtlh;
   const uint64_t sector = mode == 8 ? 4 : ((mode == 4 || mode == 5) ? 32 : 64);
-  if (!d_buf || !d_sink || mode > 8 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
+  if (!d_buf || !d_sink || mode > 9 || buf_bytes < 2 * sector || (buf_bytes & (buf_bytes - 1)))
     return DTL_EINVAL;
-  if (mode == 8 && (buf_bytes < 64 || buf_bytes > (128u << 10))) return DTL_EINVAL;
+  if (mode >= 8 && (buf_bytes < 64 || buf_bytes > (128u << 10))) return DTL_EINVAL;
   if ((n_reads + kGatherPerLane - 1) / kGatherPerLane * 4 / 256 > 0x7FFFFFFFull) return DTL_EINVAL;
   const uint64_t sectors = buf_bytes / sector;
   const uint32_t shift = 32u - (63u - static_cast<uint32_t>(__builtin_clzll(sectors)));
@@ -127,19 +148,20 @@ extern "C" int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t 
   (void)hipGetLastError();
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
-  if (mode == 8) {
+  if (mode >= 8) {
     int dev = 0, cus = 0;
     uint32_t max_lds = 0;
     const int rc = current_device(&dev, &cus, &max_lds);
     if (rc != DTL_OK) return rc;
     const uint32_t bytes = static_cast<uint32_t>(buf_bytes);
     if (bytes > max_lds) return DTL_EINVAL;
-    e = allow_dynamic_lds(reinterpret_cast<const void*>(k_lds_gather), bytes);
+    auto fn = mode == 9 ? k_lds_gather<true> : k_lds_gather<false>;
+    e = allow_dynamic_lds(reinterpret_cast<const void*>(fn), bytes);
     int per_cu = 0;
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lds_gather, 1024, bytes);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 1024, bytes);
     if (e == hipSuccess) {
       const uint32_t grid = static_cast<uint32_t>(std::max(per_cu, 1) * std::max(cus, 1));
-      hipLaunchKernelGGL(k_lds_gather, dim3(grid), dim3(1024), bytes, st, static_cast<const u32x4*>(d_buf),
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), bytes, st, static_cast<const u32x4*>(d_buf),
                          bytes, shift, n_reads, d_sink);
       e = hipGetLastError();
     }

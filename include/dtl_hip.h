@@ -342,9 +342,10 @@ int dtl_mlp_set_params(dtl_mlp* mlp, const float* h_params);
  * measure the random-access bandwidth the probe kernels are compared with).
  * Reads n_reads pseudo-random aligned sectors (64 B, or 32 B for modes 4-5)
  * of a buffer of buf_bytes (a power of two); the access pattern `mode` is
- * described in dtl_gather.hip.  Mode 8 stages the buffer (64 B .. 128 KiB)
- * in every workgroup's LDS and reads random 4-byte words of it.  d_sink is a
- * device uint64_t scratch word. */
+ * described in dtl_gather.hip.  Modes 8 and 9 stage the buffer (64 B ..
+ * 128 KiB, 128 B .. for mode 9) in every workgroup's LDS and read random
+ * 4-byte words (8) or random 64 B blocks (9) of it.  d_sink is a device
+ * uint64_t scratch word. */
 int dtl_gather_bench(const void* d_buf, uint64_t buf_bytes, uint32_t mode, uint64_t n_reads,
                      uint64_t* d_sink, dtl_stream_t stream);
 

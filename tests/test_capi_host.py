@@ -177,9 +177,11 @@ def test_gather_bench_rejects_bad_arguments():
     L = D.load()
     fake = ctypes.c_void_p(4096)
     sink = ctypes.c_void_p(8192)
-    assert L.dtl_gather_bench(fake, 1 << 20, 9, 1024, sink, None) == D.EINVAL   # modes 0..8
-    assert L.dtl_gather_bench(fake, 1 << 20, 8, 1024, sink, None) == D.EINVAL   # LDS mode: <= 128 KiB
+    assert L.dtl_gather_bench(fake, 1 << 20, 10, 1024, sink, None) == D.EINVAL  # modes 0..9
+    assert L.dtl_gather_bench(fake, 1 << 20, 8, 1024, sink, None) == D.EINVAL   # LDS modes: <= 128 KiB
+    assert L.dtl_gather_bench(fake, 1 << 20, 9, 1024, sink, None) == D.EINVAL
     assert L.dtl_gather_bench(fake, 32, 8, 1024, sink, None) == D.EINVAL        # LDS mode: >= 64 B
+    assert L.dtl_gather_bench(fake, 64, 9, 1024, sink, None) == D.EINVAL        # two 64 B blocks at least
     assert L.dtl_gather_bench(fake, (1 << 20) + 64, 0, 1024, sink, None) == D.EINVAL  # not a power of two
     assert L.dtl_gather_bench(None, 1 << 20, 0, 1024, sink, None) == D.EINVAL
 

@@ -209,3 +209,39 @@ def test_learned_choice_and_persistence(small_model, tmp_path):
             assert a.key() == b.key() and a.m == b.m and a.cost_ns == b.cost_ns
     L.close()
     os.remove(path)
+
+
+def regret_stats(M, lns, tws_cycles, clock_ghz=2.4):
+    r = []
+    for ln in lns:
+        for tc in tws_cycles:
+            tw = tc / clock_ghz
+            ml = M.choose(1 << ln, tw, use_mlp=True)
+            ex = M.choose(1 << ln, tw, use_mlp=False)
+            assert ml.cost_ns >= ex.cost_ns * (1 - 1e-12)
+            r.append(ml.cost_ns / ex.cost_ns - 1)
+    r = np.array(r)
+    return float(r.mean()), float(np.percentile(r, 95)), float((r <= 0.05).mean())
+
+
+def test_learned_model_full_calibration_regret():
+    """The full calibration sweep (279 configurations x 2^13..2^33 bits) and the
+    default training run, then the learned choice against the exhaustive
+    optimum of the same calibrated table: on the query grid the benchmark
+    reports (n = 2^10..2^28 in steps of 4x, tw = 1..1000 GPU cycles at 2.4 GHz)
+    and on a grid between its points (odd powers of two, tw between the
+    benchmark's values), where neither n nor tw was a query of the model's
+    selection.  The learned choice is priced by the table (the reference's
+    cost function, cost_fn.hpp:143-166: lookup time + tw x false positives),
+    so the regret is a property of the model, not of timing noise at query
+    time."""
+    dev()
+    M = D.CostModel.calibrate()
+    M.train()
+    on = regret_stats(M, range(10, 29, 2), (1, 2, 5, 10, 20, 50, 100, 200, 500, 1000))
+    off = regret_stats(M, range(11, 28, 2), (1.5, 3, 7, 15, 30, 70, 150, 300, 700))
+    print("regret on grid: mean %.4f p95 %.4f within5 %.3f; between: mean %.4f p95 %.4f within5 %.3f"
+          % (on + off), flush=True)
+    assert on[0] <= 0.02 and on[1] <= 0.06 and on[2] >= 0.9, on
+    assert off[0] <= 0.03 and off[1] <= 0.08 and off[2] >= 0.85, off
+    M.close()

@@ -229,6 +229,24 @@ def test_edge_sizes(n):
         assert np.array_equal(D.positions(out, cnt), exp)
 
 
+@pytest.mark.parametrize("n", [(1 << 24) - 1, 1 << 24, (1 << 24) + 1, (1 << 24) + 2049])
+def test_compaction_at_fused_scan_limit(n):
+    """Lists of up to 8192 tiles (2^24 keys) are compacted by one launch that
+    adds up the tile counts itself; longer ones by a scan and an expand.  The
+    plain register-blocked filter (LDS path) on either side of the limit,
+    against the oracle, with a match offset."""
+    cfg = (32, 1 << 20, 4, 1, 1, 1)
+    desc = D.bbf_desc(*cfg[:1], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5])
+    f = O.Bloom(*cfg)
+    build = np.arange(1, (1 << 16) + 1, dtype=np.uint32) * np.uint32(2654435761 & 0x7FFFFFFF)
+    words = gpu_build(desc, build)
+    probe = mixed_keys(build, n, 0.1, 17)
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 77)
+    torch.cuda.synchronize()
+    exp = f.batch_contains(oracle_words(f, build), probe, 77)
+    assert np.array_equal(D.positions(out, cnt), exp)
+
+
 def part_params(slice_log2):
     p = D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_PARTITIONED)
     p.slice_log2 = slice_log2
@@ -792,7 +810,7 @@ def test_batched_mixed_configurations():
 # ---------------------------------------------------------------------------
 # roofline microbenchmark: every access pattern runs (it reports only time)
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("mode,log2_bytes", [(m, 22) for m in range(8)] + [(8, 14), (8, 17)])
+@pytest.mark.parametrize("mode,log2_bytes", [(m, 22) for m in range(8)] + [(8, 14), (8, 17), (9, 14), (9, 17)])
 def test_gather_bench_modes_run(mode, log2_bytes):
     dev()
     buf = torch.randint(0, 2**31, (1 << (log2_bytes - 2),), dtype=torch.int32, device="cuda")
