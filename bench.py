@@ -488,6 +488,13 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     log("batched: autotuning the batch plan")
     plan = D.BatchPlan(tasks)
     best = plan.tune(reps=tune_reps)
+    group_shapes = []
+    for gi in range(plan.group_count()):
+        sh, nt, nk, mfb = plan.group_shape(gi)
+        group_shapes.append({"tasks": nt, "probe_keys": nk, "largest_filter_bytes": mfb,
+                             "lds_max_bytes": sh.lds_max_bytes, "block_threads": sh.block_threads,
+                             "keys_in_flight": "half" if sh.flags & D.PROBE_HALF_IN_FLIGHT else "full",
+                             "chunk_tiles": sh.chunk_tiles, "grid_blocks": sh.grid_blocks})
     batch_ms, counts = time_plan_graph(torch, plan, steps)
     plan_lists = plan.results()
     plain_lists = [D.positions(o, c) for o, c in zip(outs, cnts)]
@@ -534,7 +541,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
            "counts_identical": bool(counts == plain_counts),
            "lists_identical": bool(lists_identical),
            "tuned": {"lds_max_bytes": best.lds_max_bytes, "block_threads": best.block_threads,
-                     "chunk_tiles": best.chunk_tiles},
+                     "chunk_tiles": best.chunk_tiles, "groups": group_shapes},
            "roofline": {"bound": "hbm", "bytes_per_probe": round(hbm_bytes / n_keys, 2),
                         "achieved": round(hbm_bytes / batch_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(hbm_bytes / batch_ms / 1e6 / HBM_PEAK_GBS, 4),

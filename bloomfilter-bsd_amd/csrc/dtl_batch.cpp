@@ -60,8 +60,13 @@ void plan_free(dtl_batch_plan* p) {
 }
 
 // Side streams and fork / join events for the plan's groups 1.. (once).
+// Groups after the first run on at most kSideStreams side streams (round
+// robin), so that a plan of hundreds of groups does not create hundreds of
+// streams: HIP spreads streams over 4 hardware queues per process by default.
+constexpr size_t kSideStreams = 3;
+
 hipError_t plan_streams(dtl_batch_plan* p) {
-  const size_t need = p->groups.size() > 1 ? p->groups.size() - 1 : 0;
+  const size_t need = std::min(p->groups.size() > 1 ? p->groups.size() - 1 : 0, kSideStreams);
   if (p->side.size() == need && (need == 0 || p->fork)) return hipSuccess;
   hipError_t e = hipSuccess;
   if (!p->fork) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
@@ -164,6 +169,7 @@ int group_create(const dtl_bbf_task* tasks, const std::vector<uint32_t>& ids, co
   L.threads = threads;
   L.grid = params ? params->grid_blocks : 0;
   L.chunk = (params && params->chunk_tiles) ? params->chunk_tiles : 1;
+  L.half_in_flight = (params && (params->flags & DTL_PROBE_HALF_IN_FLIGHT)) ? 1u : 0u;
   L.cu_count = cus;
   return DTL_OK;
 }
@@ -183,7 +189,8 @@ void copy_shape(const BbfBatchLaunch& L, dtl_probe_params* out) {
   out->block_threads = L.threads;
   out->grid_blocks = L.grid;
   out->chunk_tiles = L.chunk;
-  out->flags = L.lds_max_bytes == 0 ? DTL_PROBE_FORCE_GLOBAL : 0;
+  out->flags = (L.lds_max_bytes == 0 ? DTL_PROBE_FORCE_GLOBAL : 0u) |
+               (L.half_in_flight ? DTL_PROBE_HALF_IN_FLIGHT : 0u);
 }
 
 }  // namespace
@@ -242,14 +249,15 @@ int bbf_batch_plan_run(dtl_batch_plan* p, dtl_stream_t stream) {
   (void)hipGetLastError();  // report only this call's failures
   hipError_t e = plan_streams(p);
   const size_t ng = p->groups.size();
-  if (e == hipSuccess && ng > 1) e = hipEventRecord(p->fork, st);
-  for (size_t g = 1; g < ng && e == hipSuccess; ++g) {
-    e = hipStreamWaitEvent(p->side[g - 1], p->fork, 0);
-    if (e == hipSuccess) e = group_run(p->groups[g], p->side[g - 1]);
-    if (e == hipSuccess) e = hipEventRecord(p->join[g - 1], p->side[g - 1]);
-  }
+  const size_t ns = p->side.size();
+  if (e == hipSuccess && ns > 0) e = hipEventRecord(p->fork, st);
+  for (size_t i = 0; i < ns && e == hipSuccess; ++i) e = hipStreamWaitEvent(p->side[i], p->fork, 0);
+  for (size_t g = 1; g < ng && e == hipSuccess; ++g) e = group_run(p->groups[g], p->side[(g - 1) % ns]);
   if (e == hipSuccess && ng > 0) e = group_run(p->groups[0], st);
-  for (size_t g = 1; g < ng && e == hipSuccess; ++g) e = hipStreamWaitEvent(st, p->join[g - 1], 0);
+  for (size_t i = 0; i < ns && e == hipSuccess; ++i) {
+    e = hipEventRecord(p->join[i], p->side[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, p->join[i], 0);
+  }
   if (e != hipSuccess) {
     set_last_error(e);
     return DTL_EHIP;
@@ -261,15 +269,27 @@ uint32_t bbf_batch_plan_group_count(const dtl_batch_plan* p) {
   return p ? static_cast<uint32_t>(p->groups.size()) : 0u;
 }
 
+int bbf_batch_plan_group_shape(const dtl_batch_plan* p, uint32_t group, dtl_probe_params* shape,
+                               uint32_t* n_tasks, uint64_t* keys, uint32_t* max_filter_bytes) {
+  if (!p || group >= p->groups.size()) return DTL_EINVAL;
+  const PlanGroup& g = p->groups[group];
+  if (shape) copy_shape(g.launch, shape);
+  if (n_tasks) *n_tasks = g.n_tasks;
+  if (keys) *keys = g.keys;
+  if (max_filter_bytes) *max_filter_bytes = g.max_filter_bytes;
+  return DTL_OK;
+}
+
 void bbf_batch_plan_destroy(dtl_batch_plan* p) { plan_free(p); }
 
 // Autotuner: for each group, times every candidate launch shape of the
 // persistent probe kernel on the group's own tasks (median of `reps` runs,
 // HIP events on `stream`) and keeps the fastest.  Candidates: LDS staging
 // threshold {0, the group's largest filter} x workgroup {256, 1024} threads x
-// tiles per grab {1, 2, 4, 8, 16} x grid {CUs x occupancy, half of it (at
-// least one workgroup per CU)}.  *best_out receives the shape of the group
-// with the most keys.
+// keys per lane in flight {the kernel's full count, half of it} x tiles per
+// grab {1, 2, 4, 8, 16} x grid {CUs x occupancy, half of it (at least one
+// workgroup per CU)}.  *best_out receives the shape of the group with the
+// most keys.
 int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
                         dtl_probe_params* best_out) {
   if (!p || reps == 0) return DTL_EINVAL;
@@ -299,13 +319,15 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
     for (uint32_t li = 0; li < 2; ++li) {
       const uint32_t lds = lds_opts[li];
       if (rc != DTL_OK || (li == 1 && lds == 0)) continue;
-      for (uint32_t thr : thr_opts) {
+      for (uint32_t thr : thr_opts)
+      for (uint32_t half = 0; half < 2; ++half) {
         // the full grid (CUs x occupancy) and half of it
         uint32_t full = 0;
         {
           BbfBatchLaunch probe = g.launch;
           probe.lds_max_bytes = lds;
           probe.threads = thr;
+          probe.half_in_flight = half;
           full = g.entry->batched_grid(probe);
         }
         const uint32_t grid_opts[] = {full, std::max(static_cast<uint32_t>(cus), full / 2)};
@@ -314,6 +336,7 @@ int bbf_batch_plan_tune(dtl_batch_plan* p, dtl_stream_t stream, uint32_t reps,
         for (uint32_t ch : chunk_opts) {
           g.launch.lds_max_bytes = lds;
           g.launch.threads = thr;
+          g.launch.half_in_flight = half;
           g.launch.chunk = ch;
           g.launch.grid = grid_opts[gi2];
           std::vector<float> t;

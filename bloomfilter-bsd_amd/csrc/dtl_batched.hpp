@@ -10,7 +10,9 @@
 // larger filters are probed from global memory (L2 / Infinity Cache / HBM),
 // lane-split (split_probe_tile) where the configuration allows.
 // Output is the same bitmap + per-tile count as the single-filter kernels, so
-// the compaction phases are shared.
+// the compaction phases are shared.  HALF: each lane requests half as many
+// keys (one-lane tiles) or 16-byte pieces (lane-split tiles) before it tests
+// them -- fewer registers, so more waves per SIMD; the autotuner picks.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,7 +22,7 @@
 
 namespace dtlh {
 
-template <class C, int BT>
+template <class C, int BT, bool HALF>
 __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __restrict__ tasks,
                                                           const uint32_t* __restrict__ tile_task,
                                                           uint32_t total_tiles, uint32_t chunk,
@@ -30,7 +32,8 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
                                                           uint32_t* __restrict__ tile_cnt) {
   using W = typename C::word_t;
   constexpr int ROUNDS = kTileKeys / BT;
-  constexpr int U = ROUNDS < kGlobalUnroll ? ROUNDS : kGlobalUnroll;
+  constexpr int UF = ROUNDS < kGlobalUnroll ? ROUNDS : kGlobalUnroll;
+  constexpr int U = HALF && UF > 1 ? UF / 2 : UF;  // keys per lane requested before the test
   extern __shared__ uint4 lds_filter[];
   __shared__ uint32_t red[BT / 64];
   __shared__ uint32_t s_start;
@@ -68,8 +71,8 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
       if constexpr (C::split.ok) {
         if (!use_lds) {
           // blocks of 32 B or more from L2 / HBM: several lanes per key
-          cnt = split_probe_tile<C, BT>(sl, filter, keys, n, addr, local_base,
-                                        bits + static_cast<uint64_t>(g) * kTileWords);
+          cnt = split_probe_tile<C, BT, HALF ? DTL_SPLIT_UNROLL / 2 : DTL_SPLIT_UNROLL>(
+              sl, filter, keys, n, addr, local_base, bits + static_cast<uint64_t>(g) * kTileWords);
           const uint32_t total = block_sum_waves<BT>(cnt, red);
           if (tid == 0) tile_cnt[g] = total;
           __syncthreads();
@@ -118,10 +121,11 @@ __global__ __launch_bounds__(BT) void k_bbf_probe_batched(const BatchTaskDev* __
   }
 }
 
-// CUs x workgroups per CU of the launch shape (LDS reservation, workgroup size).
+// CUs x workgroups per CU of the launch shape (LDS reservation, workgroup
+// size, keys in flight).
 template <class C, int BT>
 uint32_t batched_full_grid_bt(const BbfBatchLaunch& L) {
-  auto fn = k_bbf_probe_batched<C, BT>;
+  auto fn = L.half_in_flight ? k_bbf_probe_batched<C, BT, true> : k_bbf_probe_batched<C, BT, false>;
   const uint32_t lds_bytes = (L.lds_max_bytes + 15u) & ~15u;
   int per_cu = 0;
   if (allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes) != hipSuccess ||
@@ -140,7 +144,7 @@ uint32_t batched_full_grid(const BbfBatchLaunch& L) {
 
 template <class C, int BT>
 hipError_t launch_probe_batched_bt(const BbfBatchLaunch& L, uint32_t* work_counter, uint32_t chunk) {
-  auto fn = k_bbf_probe_batched<C, BT>;
+  auto fn = L.half_in_flight ? k_bbf_probe_batched<C, BT, true> : k_bbf_probe_batched<C, BT, false>;
   const uint32_t lds_bytes = (L.lds_max_bytes + 15u) & ~15u;
   hipError_t ae = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds_bytes);
   if (ae != hipSuccess) return ae;

@@ -66,6 +66,7 @@ struct BbfBatchLaunch {
   uint32_t threads;
   uint32_t grid;
   uint32_t chunk;              // tiles per grab
+  uint32_t half_in_flight;     // 1: half as many keys (pieces) per lane requested at once
   int cu_count;
   hipStream_t stream;
 };

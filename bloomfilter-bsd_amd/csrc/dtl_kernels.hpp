@@ -396,8 +396,8 @@ struct NoSplitLane {
 // One tile of kTileKeys keys (keys[tile_base ..], n keys in all) through the
 // lane-split test, BT threads: ballot words to bits_tile[0 .. kTileWords),
 // returns the wave's match count (uniform per wave).  `sl` is the lane's
-// SplitLane for piece = lane % L.
-template <class C, int BT>
+// SplitLane for piece = lane % L.  SU: 16-byte loads per lane per batch.
+template <class C, int BT, int SU = DTL_SPLIT_UNROLL>
 __device__ __forceinline__ uint32_t split_probe_tile(const SplitLane<C>& sl, const typename C::word_t* filter,
                                                      const uint32_t* __restrict__ keys, uint32_t n,
                                                      const AddrArgs& addr, uint64_t tile_base,
@@ -408,7 +408,7 @@ __device__ __forceinline__ uint32_t split_probe_tile(const SplitLane<C>& sl, con
   constexpr int L = P.lanes;
   constexpr int KPI = 64 / L;                   // keys per wave load instruction
   constexpr int ROUNDS = kTileKeys / BT;        // 64-key groups per wave and tile
-  constexpr int U = DTL_SPLIT_UNROLL < ROUNDS * L ? DTL_SPLIT_UNROLL : ROUNDS * L;
+  constexpr int U = SU < ROUNDS * L ? SU : ROUNDS * L;
   constexpr int RU = U / L;                     // 64-key groups per batch
   static_assert(U % L == 0 && ROUNDS % RU == 0, "unroll must cover whole 64-key groups");
   constexpr uint32_t BYTES = C::w * sizeof(W);

@@ -19,6 +19,7 @@ PROBE_DIRECT = 4
 PROBE_LANE_SPLIT = 8
 PROBE_ONE_LANE = 16
 PROBE_PART_HISTOGRAM = 32
+PROBE_HALF_IN_FLIGHT = 64
 
 
 class BbfDesc(C.Structure):
@@ -82,7 +83,7 @@ EXPORTS = [
     "cf_build_host", "cf_build", "bbf_batch_plan_create", "bbf_batch_plan_run", "bbf_batch_plan_destroy",
     "bbf_probe_batched", "bbf_batch_plan_tune", "dtl_status_string", "dtl_last_error", "dtl_bbf_supported",
     "dtl_bbf_config_count", "dtl_bbf_split_lanes", "dtl_version", "dtl_gather_bench",
-    "bbf_batch_plan_group_count", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
+    "bbf_batch_plan_group_count", "bbf_batch_plan_group_shape", "dtl_model_calibrate", "dtl_model_train", "dtl_model_choose", "dtl_model_cost", "dtl_model_lookup_ns",
     "dtl_model_fpr", "dtl_model_config_count", "dtl_model_get_config", "dtl_model_calibration_seconds",
     "dtl_model_save", "dtl_model_load", "dtl_model_destroy", "dtl_mlp_param_count", "dtl_mlp_create",
     "dtl_mlp_destroy", "dtl_mlp_train", "dtl_mlp_train_ex", "dtl_mlp_predict", "dtl_mlp_gradient", "dtl_mlp_get_params",
@@ -134,6 +135,8 @@ def load(path=LIB_PATH):
     L.dtl_gather_bench.argtypes = [vp, u64, u32, u64, vp, vp]
     L.bbf_batch_plan_group_count.argtypes = [vp]
     L.bbf_batch_plan_group_count.restype = u32
+    L.bbf_batch_plan_group_shape.argtypes = [vp, u32, C.POINTER(ProbeParams), C.POINTER(u32), C.POINTER(u64),
+                                             C.POINTER(u32)]
     f64, f32 = C.c_double, C.c_float
     L.dtl_model_calibrate.argtypes = [C.POINTER(CalibParams), C.POINTER(vp), vp]
     L.dtl_model_train.argtypes = [vp, f64, u32, u32, u32, vp]
@@ -390,6 +393,13 @@ class BatchPlan:
 
     def group_count(self):
         return self.L.bbf_batch_plan_group_count(self.plan)
+
+    def group_shape(self, group):
+        """(ProbeParams of the group's launch shape, tasks, probe keys, largest filter bytes)."""
+        shape, nt, keys, mfb = ProbeParams(), C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check("bbf_batch_plan_group_shape", self.L.bbf_batch_plan_group_shape(
+            self.plan, group, C.byref(shape), C.byref(nt), C.byref(keys), C.byref(mfb)))
+        return shape, nt.value, keys.value, mfb.value
 
     def close(self):
         if self.plan:

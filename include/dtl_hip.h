@@ -136,8 +136,11 @@ enum {
   DTL_PROBE_LANE_SPLIT = 8u,     /* always read blocks lane-split where the configuration allows
                                     (and the filter is under 4 GiB) */
   DTL_PROBE_ONE_LANE = 16u,      /* never: one lane loads a key's whole block */
-  DTL_PROBE_PART_HISTOGRAM = 32u /* partitioned probe of a blocked Bloom filter: size the partitions with
+  DTL_PROBE_PART_HISTOGRAM = 32u, /* partitioned probe of a blocked Bloom filter: size the partitions with
                                     a histogram pass over the keys instead of fixed-capacity bins */
+  DTL_PROBE_HALF_IN_FLIGHT = 64u  /* batched probe: each lane requests half as many keys (or 16-byte
+                                    pieces of lane-split blocks) before testing them: 4 instead of 8
+                                    keys, 8 instead of 16 pieces (fewer registers, more waves) */
 };
 
 /* ---- descriptors -------------------------------------------------------- */
@@ -242,8 +245,8 @@ typedef struct dtl_bbf_task {
  * memset, so a run can be captured into a hipGraph and replayed.  Tasks may
  * mix configurations (word_bits, w, s, z, k) and filter sizes: they are
  * grouped by configuration and size class in order of first appearance.
- * The first group runs on `stream`, the others on the plan's own side
- * streams (default priority, created on the first run), forked from
+ * The first group runs on `stream`, the others round robin on up to three
+ * side streams of the plan (default priority, created on the first run), forked from
  * `stream` by an event and joined back into it before the call returns, so
  * the run is ordered on `stream` as a whole (under capture the fork and join
  * are graph edges).  Task buffers are referenced, not copied: their contents
@@ -256,11 +259,17 @@ int bbf_batch_plan_run(dtl_batch_plan* plan, dtl_stream_t stream);
 void bbf_batch_plan_destroy(dtl_batch_plan* plan);
 /* Number of configuration groups (persistent launches per run). */
 uint32_t bbf_batch_plan_group_count(const dtl_batch_plan* plan);
+/* Launch shape of group `group` (as set at creation or by the autotuner), its
+ * task count, probe keys and largest filter.  No reference counterpart (plan
+ * introspection). */
+int bbf_batch_plan_group_shape(const dtl_batch_plan* plan, uint32_t group, dtl_probe_params* shape,
+                               uint32_t* n_tasks, uint64_t* keys, uint32_t* max_filter_bytes);
 
 /* Autotunes each group's launch shape (LDS staging threshold, workgroup
- * size, tiles per grab) by timing `reps` runs of each candidate on the
- * group's own tasks; the fastest shapes are kept for later runs and the one
- * of the group with the most keys is copied to *best. */
+ * size, keys per lane in flight, tiles per grab, grid) by timing `reps` runs
+ * of each candidate on the group's own tasks; the fastest shapes are kept
+ * for later runs and the one of the group with the most keys is copied to
+ * *best. */
 int bbf_batch_plan_tune(dtl_batch_plan* plan, dtl_stream_t stream, uint32_t reps,
                         dtl_probe_params* best);
 

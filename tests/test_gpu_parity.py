@@ -629,6 +629,34 @@ def test_batched_matches_per_filter(params):
     plan.close()
 
 
+def test_batched_every_configuration():
+    """One plan with a task of every configuration the reference dispatches
+    (one group each), POW2 and MAGIC sizes, ragged batches: staged in LDS and
+    from global memory, each with the full and the halved number of keys
+    (pieces) per lane in flight; every list equals the oracle's."""
+    tasks, expect = [], []
+    for i, (wb, w, s, z, k) in enumerate(all_configs()):
+        m = (1 << 16) if i % 2 == 0 else 70001
+        cfg = (wb, m, k, w, s, z)
+        f = O.Bloom(*cfg)
+        desc = D.bbf_desc(*cfg)
+        rng = np.random.default_rng(3000 + i)
+        build = rng.integers(0, 2**31, 1500, dtype=np.uint64).astype(np.uint32)
+        words = gpu_build(desc, build)
+        probe = mixed_keys(build, 2048 * (1 + i % 3) + 7 * i, 0.3, 3500 + i)
+        tasks.append((desc, words, to_dev(probe), i))
+        expect.append(f.batch_contains(oracle_words(f, build), probe, i))
+    for flags in (0, D.PROBE_HALF_IN_FLIGHT, D.PROBE_FORCE_GLOBAL, D.PROBE_FORCE_GLOBAL | D.PROBE_HALF_IN_FLIGHT):
+        plan = D.BatchPlan(tasks, D.ProbeParams(0, 0, 0, 1, flags))
+        assert plan.group_count() == len(tasks)
+        plan.run()
+        torch.cuda.synchronize()
+        for j, (g, e) in enumerate(zip(plan.results(), expect)):
+            assert np.array_equal(g, e), (tasks[j][0].word_bits, tasks[j][0].w, tasks[j][0].s, tasks[j][0].z,
+                                          tasks[j][0].k, flags)
+        plan.close()
+
+
 def test_batched_graph_replay():
     """The batch plan runs under hipGraph capture; replays match the eager run."""
     rng = np.random.default_rng(5)
@@ -797,14 +825,20 @@ def test_batched_mixed_configurations():
             assert np.array_equal(g, e)
     plan.close()
     # every filter from global memory: lane-split groups (u64 w8 s8, zoned u64
-    # w8 z4) and one-lane groups, 256- and 1024-thread workgroups
+    # w8 z4) and one-lane groups, 256- and 1024-thread workgroups, with the
+    # full and the halved number of keys (pieces) per lane in flight; and the
+    # halved count with filters staged in LDS
     for threads in (256, 1024):
-        plan = D.BatchPlan(tasks, D.ProbeParams(0, threads, 0, 2, D.PROBE_FORCE_GLOBAL))
-        plan.run()
-        torch.cuda.synchronize()
-        for g, e in zip(plan.results(), expect):
-            assert np.array_equal(g, e)
-        plan.close()
+        for flags in (D.PROBE_FORCE_GLOBAL, D.PROBE_FORCE_GLOBAL | D.PROBE_HALF_IN_FLIGHT, D.PROBE_HALF_IN_FLIGHT):
+            plan = D.BatchPlan(tasks, D.ProbeParams(0, threads, 0, 2, flags))
+            for gi in range(plan.group_count()):
+                sh = plan.group_shape(gi)[0]
+                assert bool(sh.flags & D.PROBE_HALF_IN_FLIGHT) == bool(flags & D.PROBE_HALF_IN_FLIGHT)
+            plan.run()
+            torch.cuda.synchronize()
+            for g, e in zip(plan.results(), expect):
+                assert np.array_equal(g, e), (threads, flags)
+            plan.close()
 
 
 # ---------------------------------------------------------------------------
