@@ -7,6 +7,8 @@ the whole output compared with the oracle.
   2^28 probe keys (1% hits) compared element by element with the oracle's
   threaded batch_contains (cpu_baseline.bbf_probe, pinned to the scalar
   oracle by tests/test_cpu_baseline.py), default and direct paths;
+* the longest batches: 2^31 + 2^20 + 5 keys against a 64 MiB filter with a
+  match offset that wraps positions past 2^32, default and direct paths;
 * cuckoo filter (16-bit tags, 4 slots, 2^26 buckets) loaded to 95% by the
   host build, 2^28 probes at 1% and 50% hits, default (two-pass
   partitioned) and direct paths against the oracle's threaded probe;
@@ -92,6 +94,53 @@ def test_sectorized_filter_full_batch(log2_m, threads):
     planted = hp[hp < (1 << 31)]          # probe keys drawn from the build set (absent ones have bit 31 set)
     assert len(planted) == int(round(0.01 * n_probe))
     assert len(f.batch_contains(ow, planted[:100000])) == min(len(planted), 100000)
+
+
+def test_batch_beyond_2_31_keys(threads):
+    """The longest batches: 2^31 + 2^20 + 5 keys against a 64 MiB filter, so
+    that positions pass 2^31 and the partitioned probe sizes its partitions
+    with the histogram pass (fixed-capacity bins stop at 2^30 keys), with a
+    match offset of 2^31 + 12345 so that i + match_offset wraps past 2^32 as
+    in the reference's 32-bit arithmetic (blocked_bloomfilter_batch_dispatch.hpp:
+    47-67): the default (partitioned) and the direct path against the
+    oracle's threaded list, element by element."""
+    log2_m = 29
+    cfg = (64, 1 << log2_m, 8, 8, 8, 1)
+    desc = D.bbf_desc(*cfg)
+    f = O.Bloom(*cfg)
+    n_probe = (1 << 31) + (1 << 20) + 5
+    off = (1 << 31) + 12345
+    say("fullsize: 2^31 + 2^20 + 5 keys, building")
+    # bench.make_inputs draws the hit positions with randperm, which stops at
+    # 2^31 - 1 elements: here they are drawn with replacement (about 1% hits)
+    n_build = (1 << log2_m) // 12
+    build = bench.distinct_keys(torch, n_build, 6, 0)
+    probe = bench.distinct_keys(torch, n_probe, 7, 1)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    hits = n_probe // 100
+    idx = torch.randint(0, n_probe, (hits,), device="cuda", generator=g)
+    probe[idx] = build[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
+    del idx
+    words = D.new_filter(desc)
+    D.bbf_build(desc, words, build)
+    torch.cuda.synchronize()
+    ow = CB.bbf_build(f, host_u32(build), threads)
+    assert np.array_equal(words.cpu().numpy().view(np.uint64), ow), "filter words differ"
+    del build
+    say("fullsize: 2^31 + 2^20 + 5 keys, oracle")
+    exp = cpu_list(CB.bbf_probe, f, ow, host_u32(probe), threads, off) & np.uint64(0xFFFFFFFF)
+    assert len(exp) >= int(0.009 * n_probe)
+    out = torch.empty(n_probe, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for flags in (0, D.PROBE_DIRECT):
+        say("fullsize: 2^31 + 2^20 + 5 keys, probe flags %d" % flags)
+        D.bbf_probe(desc, words, probe, off, params=D.ProbeParams(0, 0, 0, 0, flags), out=out, count=cnt)
+        torch.cuda.synchronize()
+        got = D.positions(out, cnt).astype(np.uint64)
+        assert len(got) == len(exp), (flags, len(got), len(exp))
+        assert np.array_equal(got, exp), flags
+    del out, probe, words
+    torch.cuda.empty_cache()
 
 
 def test_cuckoo_benchmark_table_full_batch(threads):
