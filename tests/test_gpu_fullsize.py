@@ -7,8 +7,9 @@ the whole output compared with the oracle.
   2^28 probe keys (1% hits) compared element by element with the oracle's
   threaded batch_contains (cpu_baseline.bbf_probe, pinned to the scalar
   oracle by tests/test_cpu_baseline.py), default and direct paths;
-* the longest batches: 2^31 + 2^20 + 5 keys against a 64 MiB filter with a
-  match offset that wraps positions past 2^32, default and direct paths;
+* the longest batches: 2^31 + 2^20 + 5 keys against a 64 MiB Bloom filter
+  and a 16 MiB cuckoo table, with a match offset that wraps positions past
+  2^32, default and direct paths;
 * cuckoo filter (16-bit tags, 4 slots, 2^26 buckets) loaded to 95% by the
   host build, 2^28 probes at 1% and 50% hits, default (two-pass
   partitioned) and direct paths against the oracle's threaded probe;
@@ -140,6 +141,47 @@ def test_batch_beyond_2_31_keys(threads):
         assert len(got) == len(exp), (flags, len(got), len(exp))
         assert np.array_equal(got, exp), flags
     del out, probe, words
+    torch.cuda.empty_cache()
+
+
+def test_cuckoo_batch_beyond_2_31_keys(threads):
+    """The same batch length for the cuckoo probe: 2^31 + 2^20 + 5 keys
+    (about 30% hits) against a 16 MiB table (2^18 buckets of 16-bit tags x 4
+    slots) at 95% load from the host build, match offset 2^31 + 12345: the
+    default two-pass partitioned probe (partitions sized by one histogram
+    pass at this length; a match list of up to 2n positions before the hit
+    sort merges the passes) and the direct probe against the oracle's
+    threaded list (cuckoofilter_logic.hpp:390-479)."""
+    log2_buckets = 18
+    desc = D.cf_desc((1 << log2_buckets) * 64, 16, 4)
+    cf = O.Cuckoo(desc.m, 16, 4)
+    n_build = int(0.95 * (1 << log2_buckets) * 4)
+    keys = bench.cuckoo_build_keys(n_build)
+    buf, inserted = D.cf_build_host(desc, keys, seed=2)
+    assert inserted == n_build
+    dbuf = torch.from_numpy(buf).cuda()
+    build_dev = torch.from_numpy(keys.view(np.int32)).cuda()
+    n = (1 << 31) + (1 << 20) + 5
+    off = (1 << 31) + 12345
+    say("fullsize: cuckoo, 2^31 + 2^20 + 5 keys")
+    probe = bench.distinct_keys(torch, n, 78, 1)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    hits = int(0.3 * n)
+    idx = torch.randint(0, n, (hits,), device="cuda", generator=g)
+    probe[idx] = build_dev[torch.randint(0, n_build, (hits,), device="cuda", generator=g)]
+    del idx
+    exp = cpu_list(CB.cf_probe, cf, buf, host_u32(probe), threads, off) & np.uint64(0xFFFFFFFF)
+    assert len(exp) >= int(0.25 * n)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for flags in (0, D.PROBE_DIRECT):
+        say("fullsize: cuckoo, 2^31 + 2^20 + 5 keys, probe flags %d" % flags)
+        D.cf_probe(desc, dbuf, probe, off, out=out, count=cnt, params=D.ProbeParams(0, 0, 0, 0, flags))
+        torch.cuda.synchronize()
+        got = D.positions(out, cnt).astype(np.uint64)
+        assert len(got) == len(exp), (flags, len(got), len(exp))
+        assert np.array_equal(got, exp), flags
+    del out, probe, dbuf
     torch.cuda.empty_cache()
 
 
