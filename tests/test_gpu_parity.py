@@ -771,6 +771,18 @@ def test_batched_autotune_keeps_results():
     plan = D.BatchPlan(tasks)
     best = plan.tune(reps=1)
     assert best.block_threads in (256, 1024) and best.chunk_tiles >= 1
+    # every group's tuned shape is one of the tuner's candidates; the counts add up
+    n_tasks = n_keys = 0
+    for gi in range(plan.group_count()):
+        sh, nt, nk, mfb = plan.group_shape(gi)
+        assert sh.block_threads in (256, 1024) and sh.chunk_tiles in (1, 2, 4, 8, 16) and sh.grid_blocks >= 1
+        assert sh.lds_max_bytes in (0, (mfb + 15) // 16 * 16)
+        assert (sh.flags & ~(D.PROBE_FORCE_GLOBAL | D.PROBE_HALF_IN_FLIGHT)) == 0
+        n_tasks += nt
+        n_keys += nk
+    assert n_tasks == len(tasks) and n_keys == sum(int(t[2].numel()) for t in tasks)
+    with pytest.raises(D.DtlError):
+        plan.group_shape(plan.group_count())
     plan.run()
     torch.cuda.synchronize()
     for g, e in zip(plan.results(), expect):
