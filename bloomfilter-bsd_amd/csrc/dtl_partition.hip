@@ -43,12 +43,10 @@ namespace dtlh {
 constexpr int kHistThreads = 256;
 constexpr int kHistUnroll = 8;
 constexpr int kHitUnroll = 8;  // positions in flight per thread in the hit sort
-#ifndef DTL_SCATTER_KPT
-#define DTL_SCATTER_KPT 16  // 32 (32K-key tiles, 134 KiB of LDS) measured 2.4% slower at 1 GiB, 9% at 128 MiB
-#endif
-constexpr int kScatterKpt = DTL_SCATTER_KPT;  // keys per thread of the scatter (tile = threads * kScatterKpt)
-constexpr uint32_t kScatterKptLog2 = kScatterKpt == 32 ? 5u : 4u;
-static_assert(kScatterKpt == 16 || kScatterKpt == 32, "16 or 32 keys per thread");
+// Keys per thread of the scatter (tile = threads x keys per thread): 16, and
+// 32 for blocked Bloom filters of 512 to 1024 partitions (partition_keys).
+// The histogram form always uses 16 (its lanes follow the scatter's tiles).
+constexpr uint32_t kHistKptLog2 = 4;
 
 constexpr int kPartScanThreads = 1024;
 
@@ -243,14 +241,14 @@ __device__ __forceinline__ uint32_t spill_slot(uint32_t* __restrict__ cursor, ui
   return 0xFFFFFFFFu;
 }
 
-template <int kScatterThreads, int KIND, bool CAP>
-__global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt)) void k_part_scatter(const uint32_t* __restrict__ keys,
+template <int kScatterThreads, int KPT, int KIND, bool CAP>
+__global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, PartFn pf, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
                                                                   uint32_t* __restrict__ keys_p,
                                                                   uint32_t* __restrict__ pos_p,
                                                                   uint32_t bin_cap) {
-  constexpr uint32_t kScatterTile = kScatterThreads * kScatterKpt;
+  constexpr uint32_t kScatterTile = kScatterThreads * KPT;
   extern __shared__ uint32_t sm[];
   __shared__ uint32_t red[kScatterThreads / 64];
   uint32_t* hist = sm;
@@ -265,14 +263,14 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
   const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
   for (uint32_t i = tid; i < nparts; i += kScatterThreads) hist[i] = 0u;
   __syncthreads();
-  uint32_t key[kScatterKpt], slot[kScatterKpt];
+  uint32_t key[KPT], slot[KPT];
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
     key[r] = load_key(keys, tile_base + (j < cnt ? j : cnt - 1));
   }
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
     const uint32_t pid = part_of<KIND>(key[r], pf);
     // rank in the partition (low 16 bits, tile < 2^16) and partition id
@@ -302,22 +300,22 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
     run += c;
   }
   __syncthreads();
-  uint32_t ps[kScatterKpt / 2];  // LDS slots (< 2^16), two per register
+  uint32_t ps[KPT / 2];  // LDS slots (< 2^16), two per register
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
     const uint32_t sv = lbase[slot[r] >> 16] + (slot[r] & 0xFFFFu);
     if (j < cnt) buf[sv] = key[r];
     ps[r / 2] = (r & 1) ? (ps[r / 2] | (sv << 16)) : sv;
   }
   __syncthreads();
-  uint32_t dst[kScatterKpt];
+  uint32_t dst[KPT];
   if constexpr (CAP) {
     if (tid == 0) s_nspill = 0u;
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
     if (sl < cnt) {
       const uint32_t k = buf[sl];
@@ -357,7 +355,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
       __syncthreads();
       const uint32_t nseg = s_nseg, placed = s_placed;
 #pragma unroll
-      for (int r = 0; r < kScatterKpt; ++r) {
+      for (int r = 0; r < KPT; ++r) {
         const uint32_t sl = r * kScatterThreads + tid;
         if (sl < cnt && (dst[r] & kSpillMark)) {
           const uint32_t idx = dst[r] & ~kSpillMark;
@@ -377,13 +375,13 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * kScatterKpt
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
     if (j < cnt) buf[(ps[r / 2] >> (16 * (r & 1))) & 0xFFFFu] = static_cast<uint32_t>(tile_base) + j;
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kScatterKpt; ++r) {
+  for (int r = 0; r < KPT; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
     if (sl < cnt && (!CAP || dst[r] != 0xFFFFFFFFu)) pos_p[dst[r]] = buf[sl];
   }
@@ -558,14 +556,14 @@ static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile, bool cap)
   return nparts * (cap ? 16u : 12u) + tile * 4u;
 }
 
-template <int BT, int KIND, bool CAP>
+template <int BT, int KPT, int KIND, bool CAP>
 static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
-  constexpr uint32_t tile = BT * kScatterKpt;
+  constexpr uint32_t tile = BT * KPT;
   const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile, CAP);
-  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KIND, CAP>), lds);
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KPT, KIND, CAP>), lds);
   if (e != hipSuccess) return e;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
-  hipLaunchKernelGGL((k_part_scatter<BT, KIND, CAP>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf,
+  hipLaunchKernelGGL((k_part_scatter<BT, KPT, KIND, CAP>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf,
                      L.nparts, L.cursor, L.keys_p, L.pos_p, L.bin_cap);
   return hipGetLastError();
 }
@@ -585,7 +583,7 @@ static PartShape part_shape(const BbfPartLaunch& L) {
   S.per_wg = static_cast<uint32_t>(per_wg);
   S.hist_grid = static_cast<uint32_t>((L.n + per_wg - 1) / per_wg);
   S.threads = L.scatter_threads >= 1024 ? 1024u : L.scatter_threads >= 512 ? 512u : 256u;
-  S.tile_log2 = (S.threads == 1024 ? 10u : S.threads == 512 ? 9u : 8u) + kScatterKptLog2;
+  S.tile_log2 = (S.threads == 1024 ? 10u : S.threads == 512 ? 9u : 8u) + kHistKptLog2;
   return S;
 }
 
@@ -595,9 +593,9 @@ static hipError_t scan_and_scatter(const BbfPartLaunch& L, const PartFn& pf, con
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, L.stream, L.counts, L.nparts * kLanes,
                      L.offsets, L.cursor);
   switch (S.threads) {
-    case 1024: return launch_scatter<1024, KIND, false>(L, pf);
-    case 512: return launch_scatter<512, KIND, false>(L, pf);
-    default: return launch_scatter<256, KIND, false>(L, pf);
+    case 1024: return launch_scatter<1024, 16, KIND, false>(L, pf);
+    case 512: return launch_scatter<512, 16, KIND, false>(L, pf);
+    default: return launch_scatter<256, 16, KIND, false>(L, pf);
   }
 }
 
@@ -631,13 +629,23 @@ hipError_t partition_keys(const BbfPartLaunch& L) {
                        L.bin_cap);
     const uint32_t thr = part_shape(L).threads;
     if (L.part_kind == 1) {
-      return thr == 1024 ? launch_scatter<1024, 1, true>(L, pf)
-             : thr == 512 ? launch_scatter<512, 1, true>(L, pf)
-                          : launch_scatter<256, 1, true>(L, pf);
+      return thr == 1024 ? launch_scatter<1024, 16, 1, true>(L, pf)
+             : thr == 512 ? launch_scatter<512, 16, 1, true>(L, pf)
+                          : launch_scatter<256, 16, 1, true>(L, pf);
     }
-    return thr == 1024 ? launch_scatter<1024, 0, true>(L, pf)
-           : thr == 512 ? launch_scatter<512, 0, true>(L, pf)
-                        : launch_scatter<256, 0, true>(L, pf);
+    // 32K-key tiles (32 keys per thread, one workgroup per CU) for blocked
+    // Bloom filters of 512..1024 partitions: the runs per bin double to ~64
+    // keys, fewer partial lines; measured on one box, interleaved: 1 GiB
+    // (512 partitions) 2.443 against 2.485 ms per call, but 128 MiB (64
+    // partitions) 2.18 against 2.06 ms and the 512 MiB cuckoo table (256) 4.80
+    // against 4.59 ms, where the runs are long already and the second
+    // workgroup per CU counts more.  1024 partitions is the most whose
+    // cursors and run bases fit in LDS beside the 128 KiB tile.
+    if (thr == 1024 && L.cf_which == 0 && L.nparts >= 512 && L.nparts <= 1024)
+      return launch_scatter<1024, 32, 0, true>(L, pf);
+    return thr == 1024 ? launch_scatter<1024, 16, 0, true>(L, pf)
+           : thr == 512 ? launch_scatter<512, 16, 0, true>(L, pf)
+                        : launch_scatter<256, 16, 0, true>(L, pf);
   }
   return L.part_kind == 1 ? partition_keys_kind<1>(L, pf) : partition_keys_kind<0>(L, pf);
 }

@@ -276,6 +276,9 @@ BIN_CASES = [
     ((64, 1 << 25, 8, 8, 8, 1), 19, 1 << 20),        # 4 MiB, 512 KiB slices: 64 bins of 8 tiles
     ((32, 1 << 25, 4, 1, 1, 1), 18, (1 << 20) + 777),  # register-blocked u32 w1, ragged last tile
     ((32, 30000011, 6, 16, 16, 2), 18, 1200001),     # zoned MAGIC, lane-split spanning zones
+    # 512 and 1024 partitions: the scatter's 32K-key tiles (32 keys per thread)
+    ((64, 1 << 25, 8, 8, 8, 1), 13, (1 << 25) + 777),
+    ((64, 1 << 25, 8, 8, 8, 1), 12, (1 << 26) + 4097),
 ]
 
 
@@ -293,11 +296,13 @@ def test_partitioned_capacity_bins_match_histogram_and_oracle(cfg, slice_log2, n
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("dup_share", [0.3, 0.9])
-def test_partitioned_capacity_bins_spill_exact(dup_share):
+@pytest.mark.parametrize("dup_share,slice_log2,n", [(0.3, 19, 1 << 21), (0.9, 19, 1 << 21),
+                                                   (0.5, 13, (1 << 25) + 5)])
+def test_partitioned_capacity_bins_spill_exact(dup_share, slice_log2, n):
     """A batch in which most keys are one value (all in one partition) fills
     that partition's bins far past their capacity: the keys that do not fit
-    take slots of other bins, and the list still equals the oracle's."""
+    take slots of other bins, and the list still equals the oracle's (8
+    partitions; and 512, where the scatter takes 32K-key tiles)."""
     cfg = (64, 1 << 25, 8, 8, 8, 1)
     f = O.Bloom(*cfg)
     desc = D.bbf_desc(*cfg)
@@ -305,13 +310,12 @@ def test_partitioned_capacity_bins_spill_exact(dup_share):
     build = rng.integers(0, 2**31, 60000, dtype=np.uint64).astype(np.uint32)
     words = gpu_build(desc, build)
     ow = oracle_words(f, build)
-    n = 1 << 21
     probe = mixed_keys(build, n, 0.05, 63)
     idx = rng.choice(n, int(dup_share * n), replace=False)
     probe[idx[: len(idx) // 2]] = build[7]            # a member: every copy matches
     probe[idx[len(idx) // 2:]] = np.uint32(2**31 + 5)  # a non-member
     exp = f.batch_contains(ow, probe, 11)
-    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 11, params=part_params(19))
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 11, params=part_params(slice_log2))
     torch.cuda.synchronize()
     assert np.array_equal(D.positions(out, cnt), exp)
 
