@@ -779,18 +779,31 @@ int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys,
     // both passes' partition counts from one read of the keys (pass 1), when
     // the two bin arrays fit in a workgroup's LDS
     const bool one_hist = nparts <= 1024;
+    // With capacity bins, pass 2 leaves out the keys pass 1 found once they
+    // are more than n/32 (a device-side test, no host synchronisation): pass
+    // 1's matches are sorted into the key-order bitmap first, pass 2 scatters
+    // only keys whose bit is clear, and its matches are ORed into that
+    // bitmap.  Below the threshold the early sort's kernels return at once
+    // and both passes' matches are sorted together.
+    const bool skip2 = bin_cap != 0 && !(flags & DTL_PROBE_CF_PASS2_ALL);
+    const uint64_t gate_thr = n / 32;
     for (uint32_t pass = 1; pass <= 2 && he == hipSuccess; ++pass) {
       P.part_kind = pass - 1;
       P.cf_which = pass;
       P.counts = pass == 1 ? counts1 : counts2;
       P.counts_next = (pass == 1 && one_hist) ? counts2 : nullptr;
       P.counts_ready = (pass == 2 && one_hist) ? 1 : 0;
+      P.skip_bits = (pass == 2 && skip2) ? P.bits : nullptr;
+      P.gate_count = (pass == 2 && skip2) ? d_h1 : nullptr;
+      P.gate_thr = gate_thr;
       he = partition_keys(P);
       if (he == hipSuccess) he = e->probe_part(P);
       if (he == hipSuccess) he = part_gather_matches(P, hp, pass == 1 ? nullptr : d_h1, pass == 1 ? d_h1 : d_h2);
+      if (he == hipSuccess && pass == 1 && skip2) he = part_sort_matches(P, hp, d_h1, kHitSortEarly, d_h1, gate_thr);
     }
     if (he == hipSuccess) he = add_counts(d_h1, d_h2, P.d_hits, st);
-    if (he == hipSuccess) he = part_sort_matches(P, hp, P.d_hits);
+    if (he == hipSuccess)
+      he = skip2 ? part_sort_matches(P, hp, P.d_hits, kHitSortRest, d_h1, gate_thr) : part_sort_matches(P, hp, P.d_hits);
     if (he != hipSuccess) return hip_status(he);
     if (!ev(params ? params->ev_stop : nullptr)) return DTL_EHIP;
     return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));

@@ -120,7 +120,22 @@ struct BbfPartLaunch {
   int split;              // lane-split probe kernel where the configuration has one
   int cu_count;
   hipStream_t stream;
+  // Cuckoo pass 2 with capacity bins: when the device count *gate_count of
+  // pass-1 matches exceeds gate_thr, the scatter leaves out every key whose
+  // bit is set in skip_bits (key order: pass 1 found it already).
+  const uint64_t* skip_bits;
+  const uint64_t* gate_count;
+  uint64_t gate_thr;
 };
+
+// Stages of the hit sort (part_sort_matches) around a device-side gate
+// *gate > thr, read by the kernels so that no host synchronisation is needed:
+// kHitSortAll sorts hp[0 .. *d_hits) into the key-order bitmap;
+// kHitSortEarly does the same only when the gate is open (else every kernel
+// returns at once); kHitSortRest sorts hp[*gate .. *d_hits) and ORs it into
+// the bitmap the early stage wrote when the gate is open, and sorts all of
+// hp[0 .. *d_hits) as kHitSortAll when it is closed.
+enum HitSortStage : uint32_t { kHitSortAll = 0, kHitSortEarly = 1, kHitSortRest = 2 };
 
 // Hit sort of the partitioned probe: matches are bucketed by position,
 // 2^18 positions per bucket or more, so that there are at most
@@ -176,7 +191,8 @@ hipError_t part_reorder_matches(const BbfPartLaunch& L);
 // matches appended to hp at *d_base (0 if null), their number to *d_count;
 // then the hit sort of hp[0 .. *d_hits) into the key-order bitmap.
 hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint64_t* d_base, uint64_t* d_count);
-hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits);
+hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits,
+                             HitSortStage stage = kHitSortAll, const uint64_t* gate = nullptr, uint64_t thr = 0);
 hipError_t add_counts(const uint64_t* a, const uint64_t* b, uint64_t* out, hipStream_t stream);
 
 struct CfProbeLaunch {

@@ -241,13 +241,26 @@ __device__ __forceinline__ uint32_t spill_slot(uint32_t* __restrict__ cursor, ui
   return 0xFFFFFFFFu;
 }
 
-template <int kScatterThreads, int KPT, int KIND, bool CAP>
+// Keys a scatter leaves out (SKIP forms only): those whose bit is set in a
+// key-order bitmap, while a device count exceeds a threshold (the cuckoo
+// probe's pass 2 drops the keys pass 1 found, once enough of them did).
+struct KeySkip {
+  const uint64_t* bits;
+  const uint64_t* gate;
+  uint64_t thr;
+};
+
+template <int kScatterThreads, int KPT, int KIND, bool CAP, bool SKIP = false>
 __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void k_part_scatter(const uint32_t* __restrict__ keys,
                                                                   uint32_t n, PartFn pf, uint32_t nparts,
                                                                   uint32_t* __restrict__ cursor,
                                                                   uint32_t* __restrict__ keys_p,
                                                                   uint32_t* __restrict__ pos_p,
-                                                                  uint32_t bin_cap) {
+                                                                  uint32_t bin_cap, KeySkip sk) {
+  // SKIP: a left-out key's slot is kDropSlot, and its LDS slot in ps has bit
+  // 15 set (slots stay below 2^15: tiles of at most 32K keys)
+  constexpr uint32_t kDropSlot = 0xFFFFFFFFu, kDropLds = 0x8000u;
+  static_assert(kScatterThreads * KPT <= 32768, "LDS slots below 2^15");
   constexpr uint32_t kScatterTile = kScatterThreads * KPT;
   extern __shared__ uint32_t sm[];
   __shared__ uint32_t red[kScatterThreads / 64];
@@ -262,19 +275,48 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void 
   const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kScatterTile;
   const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kScatterTile, n - tile_base));
   for (uint32_t i = tid; i < nparts; i += kScatterThreads) hist[i] = 0u;
-  __syncthreads();
   uint32_t key[KPT], slot[KPT];
+  bool skip_on = false;
+  uint32_t dropm = 0;  // SKIP: bit r set = leave out this thread's key r
+  if constexpr (SKIP) {
+    // the tile's skip bits (cnt / 8 bytes) go to the front of buf, which
+    // holds nothing else until the keys are placed: one load per thread,
+    // issued before the keys, instead of one dependent load per key
+    skip_on = *sk.gate > sk.thr;  // uniform
+    const uint32_t nw = (cnt + 31u) / 32u;
+    uint32_t sw = 0;
+    if (skip_on && tid < nw)
+      __builtin_memcpy(&sw, reinterpret_cast<const char*>(sk.bits) + (tile_base / 32u + tid) * 4u, 4);
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    const uint32_t j = r * kScatterThreads + tid;
-    key[r] = load_key(keys, tile_base + (j < cnt ? j : cnt - 1));
+    for (int r = 0; r < KPT; ++r) {
+      const uint32_t j = r * kScatterThreads + tid;
+      key[r] = load_key(keys, tile_base + (j < cnt ? j : cnt - 1));
+    }
+    if (skip_on && tid < nw) buf[tid] = sw;
+    __syncthreads();
+    if (skip_on) {
+#pragma unroll
+      for (int r = 0; r < KPT; ++r) {
+        const uint32_t j = r * kScatterThreads + tid;  // words past the tile's end: read, never used
+        dropm |= ((buf[j >> 5] >> (j & 31u)) & 1u) << r;
+      }
+    }
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+      const uint32_t j = r * kScatterThreads + tid;
+      key[r] = load_key(keys, tile_base + (j < cnt ? j : cnt - 1));
+    }
   }
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
     const uint32_t pid = part_of<KIND>(key[r], pf);
+    bool drop = false;
+    if constexpr (SKIP) drop = j < cnt && ((dropm >> r) & 1u) != 0u;
     // rank in the partition (low 16 bits, tile < 2^16) and partition id
-    slot[r] = j < cnt ? (atomicAdd(&hist[pid], 1u) | (pid << 16)) : 0u;
+    slot[r] = drop ? kDropSlot : j < cnt ? (atomicAdd(&hist[pid], 1u) | (pid << 16)) : 0u;
   }
   __syncthreads();
   // exclusive scan of the histogram (each thread owns `per` consecutive bins)
@@ -300,12 +342,15 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void 
     run += c;
   }
   __syncthreads();
+  // keys in the tile's LDS buffer: all cnt, or those a SKIP form keeps
+  const uint32_t m = SKIP ? tot : cnt;
   uint32_t ps[KPT / 2];  // LDS slots (< 2^16), two per register
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
-    const uint32_t sv = lbase[slot[r] >> 16] + (slot[r] & 0xFFFFu);
-    if (j < cnt) buf[sv] = key[r];
+    const bool drop = SKIP && slot[r] == kDropSlot;
+    const uint32_t sv = drop ? kDropLds : lbase[slot[r] >> 16] + (slot[r] & 0xFFFFu);
+    if (j < cnt && !drop) buf[sv] = key[r];
     ps[r / 2] = (r & 1) ? (ps[r / 2] | (sv << 16)) : sv;
   }
   __syncthreads();
@@ -317,7 +362,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void 
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
-    if (sl < cnt) {
+    if (sl < m) {
       const uint32_t k = buf[sl];
       const uint32_t p = part_of<KIND>(k, pf);
       const uint32_t o = sl - lbase[p];
@@ -357,7 +402,7 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void 
 #pragma unroll
       for (int r = 0; r < KPT; ++r) {
         const uint32_t sl = r * kScatterThreads + tid;
-        if (sl < cnt && (dst[r] & kSpillMark)) {
+        if (sl < m && (dst[r] & kSpillMark)) {
           const uint32_t idx = dst[r] & ~kSpillMark;
           const uint32_t k = buf[sl];
           uint32_t d = 0xFFFFFFFFu;
@@ -377,13 +422,14 @@ __global__ __launch_bounds__(kScatterThreads, kScatterThreads / (8 * KPT)) void 
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t j = r * kScatterThreads + tid;
-    if (j < cnt) buf[(ps[r / 2] >> (16 * (r & 1))) & 0xFFFFu] = static_cast<uint32_t>(tile_base) + j;
+    const uint32_t sv = (ps[r / 2] >> (16 * (r & 1))) & 0xFFFFu;
+    if (j < cnt && (!SKIP || !(sv & kDropLds))) buf[sv] = static_cast<uint32_t>(tile_base) + j;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t sl = r * kScatterThreads + tid;
-    if (sl < cnt && (!CAP || dst[r] != 0xFFFFFFFFu)) pos_p[dst[r]] = buf[sl];
+    if (sl < m && (!CAP || dst[r] != 0xFFFFFFFFu)) pos_p[dst[r]] = buf[sl];
   }
 }
 
@@ -402,21 +448,40 @@ __global__ void k_part_init_bins(uint32_t* __restrict__ cursor, uint32_t nbins, 
 //   k_hit_place    writes its positions to bucket offset + hcounts[g][b] +
 //                  LDS rank (order inside a bucket is irrelevant: k_hit_bits
 //                  turns each bucket into a bitmap).
-__device__ __forceinline__ void hit_slice(const uint64_t* d_hits, uint64_t* b0, uint64_t* b1) {
-  const uint64_t h = *d_hits;
-  *b0 = h * blockIdx.x / gridDim.x;
-  *b1 = h * (blockIdx.x + 1) / gridDim.x;
+// The part of hp a hit-sort stage works on (HitSortStage): hp[first ..
+// *d_hits), first = *gate for the second stage of an open gate, else 0.
+struct HitSel {
+  const uint64_t* d_hits;
+  const uint64_t* gate;  // null for kHitSortAll
+  uint64_t thr;
+  uint32_t stage;
+};
+
+__device__ __forceinline__ bool hit_gate_open(const HitSel& s) { return s.gate != nullptr && *s.gate > s.thr; }
+
+// false when the stage has nothing to do (kHitSortEarly, gate closed); the
+// answer is the same for every thread of the grid
+__device__ __forceinline__ bool hit_stage_runs(const HitSel& s) {
+  return s.stage != kHitSortEarly || hit_gate_open(s);
 }
 
-__global__ __launch_bounds__(256) void k_hit_count(const uint32_t* __restrict__ hp,
-                                                   const uint64_t* __restrict__ d_hits, uint32_t shift,
+// this workgroup's share [b0, b1) of the stage's range of hp
+__device__ __forceinline__ void hit_slice(const HitSel& s, uint64_t* b0, uint64_t* b1) {
+  const uint64_t first = (s.stage == kHitSortRest && hit_gate_open(s)) ? *s.gate : 0;
+  const uint64_t h = *s.d_hits - first;
+  *b0 = first + h * blockIdx.x / gridDim.x;
+  *b1 = first + h * (blockIdx.x + 1) / gridDim.x;
+}
+
+__global__ __launch_bounds__(256) void k_hit_count(const uint32_t* __restrict__ hp, HitSel sel, uint32_t shift,
                                                    uint32_t nbuckets, uint32_t* __restrict__ hcounts) {
   extern __shared__ uint32_t hist[];
+  if (!hit_stage_runs(sel)) return;  // uniform, before any barrier
   const uint32_t tid = threadIdx.x;
   for (uint32_t b = tid; b < nbuckets; b += 256) hist[b] = 0u;
   __syncthreads();
   uint64_t b0, b1;
-  hit_slice(d_hits, &b0, &b1);
+  hit_slice(sel, &b0, &b1);
   // kHitUnroll positions in flight per thread, then their LDS atomics
   uint64_t i = b0 + tid;
   for (; i + (kHitUnroll - 1) * 256 < b1; i += kHitUnroll * 256) {
@@ -439,8 +504,9 @@ constexpr uint32_t kColB = 16, kColS = 16;
 static_assert(kColB * kColS == 256, "one thread per (bucket, segment)");
 
 __global__ __launch_bounds__(256) void k_hit_colscan(uint32_t* __restrict__ hcounts, uint32_t groups,
-                                                     uint32_t nbuckets, uint32_t* __restrict__ totals) {
+                                                     uint32_t nbuckets, uint32_t* __restrict__ totals, HitSel sel) {
   __shared__ uint32_t seg[kColS][kColB + 1];
+  if (!hit_stage_runs(sel)) return;
   const uint32_t bl = threadIdx.x % kColB, sg = threadIdx.x / kColB;
   const uint32_t b = blockIdx.x * kColB + bl;
   const uint32_t per = (groups + kColS - 1) / kColS;
@@ -472,12 +538,12 @@ __global__ __launch_bounds__(256) void k_hit_colscan(uint32_t* __restrict__ hcou
   }
 }
 
-__global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ hp,
-                                                   const uint64_t* __restrict__ d_hits, uint32_t shift,
+__global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ hp, HitSel sel, uint32_t shift,
                                                    uint32_t nbuckets, const uint32_t* __restrict__ hcounts,
                                                    const uint32_t* __restrict__ boffsets,
                                                    uint32_t* __restrict__ out) {
   extern __shared__ uint32_t sm[];
+  if (!hit_stage_runs(sel)) return;
   uint32_t* rank = sm;
   uint32_t* base = sm + nbuckets;
   const uint32_t tid = threadIdx.x;
@@ -488,7 +554,7 @@ __global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ 
   }
   __syncthreads();
   uint64_t b0, b1;
-  hit_slice(d_hits, &b0, &b1);
+  hit_slice(sel, &b0, &b1);
   uint64_t i = b0 + tid;
   for (; i + (kHitUnroll - 1) * 256 < b1; i += kHitUnroll * 256) {
     uint32_t p[kHitUnroll];
@@ -509,16 +575,29 @@ __global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ 
 
 // One workgroup per bucket of 2^shift positions: its matches set bits of an
 // LDS bitmap (dynamic, 2^shift bits), which is written out whole (zeros
-// included) together with the match count of each of its tiles.
+// included) together with the match count of each of its tiles.  The second
+// stage of an open gate starts from the bitmap the first stage wrote.
 __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ sorted,
                                                   const uint32_t* __restrict__ boffsets, uint32_t shift,
                                                   uint32_t ntiles, uint64_t* __restrict__ bits,
-                                                  uint32_t* __restrict__ tile_cnt) {
+                                                  uint32_t* __restrict__ tile_cnt, HitSel sel) {
   extern __shared__ uint32_t lbits[];
+  if (!hit_stage_runs(sel)) return;
+  const bool acc = sel.stage == kHitSortRest && hit_gate_open(sel);
   const uint32_t words32 = (1u << shift) / 32u;
   const uint32_t btiles = (1u << shift) / kTileKeys;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
-  for (uint32_t i = tid; i < words32; i += 256) lbits[i] = 0u;
+  const uint32_t tile0 = b * btiles;
+  const uint32_t tiles = std::min(btiles, ntiles - tile0);
+  if (acc) {
+    for (uint32_t w = tid; w < words32 / 2u; w += 256) {
+      const uint64_t v = w < tiles * kTileWords ? bits[static_cast<uint64_t>(tile0) * kTileWords + w] : 0ull;
+      lbits[2 * w] = static_cast<uint32_t>(v);
+      lbits[2 * w + 1] = static_cast<uint32_t>(v >> 32);
+    }
+  } else {
+    for (uint32_t i = tid; i < words32; i += 256) lbits[i] = 0u;
+  }
   __syncthreads();
   const uint32_t beg = boffsets[b], end = boffsets[b + 1];
   const uint32_t lmask = (1u << shift) - 1u;
@@ -535,8 +614,6 @@ __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ s
     atomicOr(&lbits[p >> 5], 1u << (p & 31u));
   }
   __syncthreads();
-  const uint32_t tile0 = b * btiles;
-  const uint32_t tiles = std::min(btiles, ntiles - tile0);
   for (uint32_t w = tid; w < tiles * kTileWords; w += 256) {
     const uint64_t v = static_cast<uint64_t>(lbits[2 * w]) | (static_cast<uint64_t>(lbits[2 * w + 1]) << 32);
     bits[static_cast<uint64_t>(tile0) * kTileWords + w] = v;
@@ -556,15 +633,20 @@ static uint32_t part_scatter_lds_bytes(uint32_t nparts, uint32_t tile, bool cap)
   return nparts * (cap ? 16u : 12u) + tile * 4u;
 }
 
-template <int BT, int KPT, int KIND, bool CAP>
+template <int BT, int KPT, int KIND, bool CAP, bool SKIP = false>
 static hipError_t launch_scatter(const BbfPartLaunch& L, const PartFn& pf) {
   constexpr uint32_t tile = BT * KPT;
   const uint32_t lds = part_scatter_lds_bytes(L.nparts, tile, CAP);
-  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_part_scatter<BT, KPT, KIND, CAP>), lds);
+  auto fn = k_part_scatter<BT, KPT, KIND, CAP, SKIP>;
+  hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(fn), lds);
   if (e != hipSuccess) return e;
+  KeySkip sk;
+  sk.bits = L.skip_bits;
+  sk.gate = L.gate_count;
+  sk.thr = L.gate_thr;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(L.n) + tile - 1) / tile);
-  hipLaunchKernelGGL((k_part_scatter<BT, KPT, KIND, CAP>), dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf,
-                     L.nparts, L.cursor, L.keys_p, L.pos_p, L.bin_cap);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(BT), lds, L.stream, L.keys, L.n, pf, L.nparts, L.cursor, L.keys_p,
+                     L.pos_p, L.bin_cap, sk);
   return hipGetLastError();
 }
 
@@ -629,6 +711,12 @@ hipError_t partition_keys(const BbfPartLaunch& L) {
                        L.bin_cap);
     const uint32_t thr = part_shape(L).threads;
     if (L.part_kind == 1) {
+      if (L.skip_bits) {
+        if (!L.gate_count) return hipErrorInvalidValue;
+        return thr == 1024 ? launch_scatter<1024, 16, 1, true, true>(L, pf)
+               : thr == 512 ? launch_scatter<512, 16, 1, true, true>(L, pf)
+                            : launch_scatter<256, 16, 1, true, true>(L, pf);
+      }
       return thr == 1024 ? launch_scatter<1024, 16, 1, true>(L, pf)
              : thr == 512 ? launch_scatter<512, 16, 1, true>(L, pf)
                           : launch_scatter<256, 16, 1, true>(L, pf);
@@ -656,21 +744,29 @@ hipError_t part_gather_matches(const BbfPartLaunch& L, uint32_t* hp, const uint6
   return e;
 }
 
-hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits) {
+hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const uint64_t* d_hits, HitSortStage stage,
+                             const uint64_t* gate, uint64_t thr) {
   const hipStream_t st = L.stream;
   const uint32_t G = L.hit_groups, nb = L.nbuckets;
+  if (stage != kHitSortAll && gate == nullptr) return hipErrorInvalidValue;
+  HitSel sel;
+  sel.d_hits = d_hits;
+  sel.gate = stage == kHitSortAll ? nullptr : gate;
+  sel.thr = thr;
+  sel.stage = stage;
   hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_place), nb * 8u);
   if (e == hipSuccess) e = allow_dynamic_lds(reinterpret_cast<const void*>(k_hit_bits), (1u << L.bucket_shift) / 8u);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, hp, d_hits, L.bucket_shift, nb, L.hcounts);
-  hipLaunchKernelGGL(k_hit_colscan, dim3((nb + kColB - 1) / kColB), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts);
+  hipLaunchKernelGGL(k_hit_count, dim3(G), dim3(256), nb * 4u, st, hp, sel, L.bucket_shift, nb, L.hcounts);
+  hipLaunchKernelGGL(k_hit_colscan, dim3((nb + kColB - 1) / kColB), dim3(256), 0, st, L.hcounts, G, nb, L.bcounts,
+                     sel);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartScanThreads), 0, st, L.bcounts, nb, L.boffsets,
                      L.bcursor);
   // bucketed positions overwrite the grouped positions
-  hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, hp, d_hits, L.bucket_shift, nb,
+  hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, hp, sel, L.bucket_shift, nb,
                      L.hcounts, L.boffsets, L.pos_p);
   hipLaunchKernelGGL(k_hit_bits, dim3(nb), dim3(256), (1u << L.bucket_shift) / 8u, st, L.pos_p, L.boffsets,
-                     L.bucket_shift, L.ntiles, L.bits, L.tile_cnt);
+                     L.bucket_shift, L.ntiles, L.bits, L.tile_cnt, sel);
   return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ PROBE_LANE_SPLIT = 8
 PROBE_ONE_LANE = 16
 PROBE_PART_HISTOGRAM = 32
 PROBE_HALF_IN_FLIGHT = 64
+PROBE_CF_PASS2_ALL = 128
 
 
 class BbfDesc(C.Structure):

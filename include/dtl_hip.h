@@ -138,9 +138,12 @@ enum {
   DTL_PROBE_ONE_LANE = 16u,      /* never: one lane loads a key's whole block */
   DTL_PROBE_PART_HISTOGRAM = 32u, /* partitioned probe of a blocked Bloom filter: size the partitions with
                                     a histogram pass over the keys instead of fixed-capacity bins */
-  DTL_PROBE_HALF_IN_FLIGHT = 64u  /* batched probe: each lane requests half as many keys (or 16-byte
+  DTL_PROBE_HALF_IN_FLIGHT = 64u, /* batched probe: each lane requests half as many keys (or 16-byte
                                     pieces of lane-split blocks) before testing them: 4 instead of 8
                                     keys, 8 instead of 16 pieces (fewer registers, more waves) */
+  DTL_PROBE_CF_PASS2_ALL = 128u   /* partitioned cuckoo probe with fixed-capacity bins: the second pass
+                                    regroups every key, also when the first pass found more than n/32
+                                    of them (by default it then leaves those keys out) */
 };
 
 /* ---- descriptors -------------------------------------------------------- */
@@ -191,7 +194,9 @@ int cf_probe(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, ui
  * use the partitioned probe: keys grouped by the slice of their first bucket, that
  * bucket and the victim cache tested from L2, then regrouped by the slice of
  * their alternate bucket and that bucket tested; ~20 bytes of scratch per key.
- * Honoured: flags DTL_PROBE_DIRECT / DTL_PROBE_PARTITIONED, slice_log2,
+ * Where the first pass finds more than n/32 keys (high hit rates), the second
+ * regroups only the keys it did not find (decided on the device).
+ * Honoured: flags DTL_PROBE_DIRECT / DTL_PROBE_PARTITIONED / DTL_PROBE_CF_PASS2_ALL, slice_log2,
  * block_threads (scatter workgroup size), ev_start / ev_stop.  No counterpart
  * of its own in the reference (cf.cpp:49-54 has no launch parameters). */
 int cf_probe_ex(const dtl_cf_desc* d, const void* d_buf, const uint32_t* d_keys, uint64_t n,

@@ -481,6 +481,42 @@ def test_cuckoo_default_path_at_break_even_size():
         assert len(got) == len(exp) and np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("hit", [0.0, 0.01, 0.03, 0.1, 0.5, 1.0, "dup"])
+def test_cuckoo_pass2_leaves_out_found_keys(hit):
+    """Partitioned cuckoo probe with fixed-capacity bins (16 partitions of an
+    8 MiB table at 95% load, 2^21 + 5 keys): where pass 1 finds more than
+    n/32 keys (about 59% of a table's keys sit in their first bucket, so hit
+    rates of 10% and more), pass 2 regroups only the keys pass 1 did not
+    find and its matches are ORed into pass 1's sorted bitmap; below that
+    both passes take every key.  Either way, and with every key regrouped
+    twice (PROBE_CF_PASS2_ALL), the list is the oracle's.  "dup": 90% of the
+    keys are one table key (pass 2 leaves nearly all out; pass 1's bins
+    spill)."""
+    m = (1 << 20) * 64
+    d = D.cf_desc(m, 16, 4)
+    o = O.Cuckoo(m, 16, 4)
+    keys = distinct_keys(int(0.95 * d.bucket_cnt * 4), 31)
+    buf, ins = D.cf_build_host(d, keys, seed=31)
+    assert ins == len(keys)
+    n = (1 << 21) + 5
+    if hit == "dup":
+        probe = mixed_keys(keys, n, 0.05, 32)
+        rng = np.random.default_rng(33)
+        probe[rng.choice(n, int(0.9 * n), replace=False)] = keys[12345]
+    else:
+        probe = mixed_keys(keys, n, hit, 32)
+    exp = o.batch_contains(buf, probe, 7)
+    dbuf = torch.from_numpy(buf).to(dev())
+    kd = to_dev(probe)
+    for flags in (D.PROBE_PARTITIONED, D.PROBE_PARTITIONED | D.PROBE_CF_PASS2_ALL):
+        prm = D.ProbeParams(0, 0, 0, 0, flags)
+        prm.slice_log2 = 19
+        for _ in range(2):  # the second call finds the first one's scratch contents
+            out, cnt = D.cf_probe(d, dbuf, kd, 7, params=prm)
+            torch.cuda.synchronize()
+            assert np.array_equal(D.positions(out, cnt), exp), (hit, flags)
+
+
 # ---------------------------------------------------------------------------
 # cuckoo GPU build: concurrent inserts give a different table than the
 # sequential insert, so parity is "valid table + same probe semantics": the
