@@ -598,7 +598,7 @@ def run_batched(torch, D, n_filters, steps, seed=11, tw_cycles=TW_CYCLES, model=
     return res
 
 
-def run_learned(torch, D, train_fraction=1.0, samples=2000000, epochs=32, model_path=None):
+def run_learned(torch, D, train_fraction=1.0, samples=2000000, epochs=64, model_path=None):
     """Calibration sweep (every candidate configuration x filter sizes 2^13..2^33
     bits, timed on the GPU; FPR measured per configuration and bits per key),
     MLP trained on a seeded `train_fraction` of the (configuration, size)

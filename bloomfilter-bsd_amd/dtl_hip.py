@@ -530,7 +530,7 @@ class CostModel:
     def save(self, path):
         _check("dtl_model_save", self.L.dtl_model_save(self.h, path.encode()))
 
-    def train(self, train_fraction=1.0, samples=2000000, epochs=32, seed=1):
+    def train(self, train_fraction=1.0, samples=2000000, epochs=64, seed=1):
         _check("dtl_model_train", self.L.dtl_model_train(self.h, train_fraction, samples, epochs, seed,
                                                          _stream(None)))
 
