@@ -138,14 +138,23 @@ static_assert(kExpandThreads * 8 == kTileKeys, "one bitmap byte per thread");
 // t0 + r; it is read before the scans, so its latency overlaps theirs.
 constexpr int kExpandGroup = 8;
 
-template <class OffOf, class Emit>
-__device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
-                                             OffOf&& off_of, uint64_t* sh, Emit&& emit) {
-  const uint32_t t = threadIdx.x;
+// The part of an expand that does not depend on where the matches go: each
+// thread's bitmap byte of every tile of the group, and its rank among the
+// group's matches (per tile: 16-bit fields, tiles 0-3 in ex[0], 4-7 in ex[1];
+// the tiles' totals in tot[0], tot[1] the same way).
+struct ExpandPrep {
   uint32_t base[kExpandGroup];
-#pragma unroll
-  for (int r = 0; r < kExpandGroup; ++r) base[r] = off_of(static_cast<uint32_t>(r) < nt ? r : nt - 1u);
   uint32_t b8[kExpandGroup];
+  uint64_t ex[2], tot[2];
+  uint32_t kb;
+};
+
+template <class OffOf>
+__device__ __forceinline__ void expand_prep(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
+                                            OffOf&& off_of, uint64_t* sh, ExpandPrep& P) {
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) P.base[r] = off_of(static_cast<uint32_t>(r) < nt ? r : nt - 1u);
   // Every tile slot loads (tiles past the group's end re-read its last tile
   // and are masked): straight-line loads, no branches around them.  A form
   // that skipped the loads of missing tiles behind branches gave wrong
@@ -156,29 +165,95 @@ __device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, 
     const uint32_t rr = static_cast<uint32_t>(r) < nt ? static_cast<uint32_t>(r) : nt - 1u;
     const uint64_t w = bits[static_cast<uint64_t>(t0 + rr) * kTileWords + (t >> 3)];
     const uint64_t word = static_cast<uint32_t>(r) < nt ? w : 0ull;
-    b8[r] = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
+    P.b8[r] = static_cast<uint32_t>(word >> ((t & 7u) * 8u)) & 0xFFu;
   }
   uint64_t lo = 0, hi = 0;
 #pragma unroll
   for (int r = 0; r < kExpandGroup; ++r) {
-    const uint64_t c = static_cast<uint64_t>(__popc(b8[r])) << (16 * (r % 4));
+    const uint64_t c = static_cast<uint64_t>(__popc(P.b8[r])) << (16 * (r % 4));
     if (r < 4) lo |= c; else hi |= c;
   }
-  uint64_t tot;
-  const uint64_t elo = block_excl_scan64<kExpandThreads>(lo, tot, sh);
-  const uint64_t ehi = block_excl_scan64<kExpandThreads>(hi, tot, sh);
-  const uint32_t kb = (t >> 3) * 64u + (t & 7u) * 8u;
+  P.ex[0] = block_excl_scan64<kExpandThreads>(lo, P.tot[0], sh);
+  P.ex[1] = block_excl_scan64<kExpandThreads>(hi, P.tot[1], sh);
+  P.kb = (t >> 3) * 64u + (t & 7u) * 8u;
+}
+
+__device__ __forceinline__ uint32_t expand_field(const uint64_t* v, int r) {
+  return static_cast<uint32_t>((v[r / 4] >> (16 * (r % 4))) & 0xFFFFu);
+}
+
+// Expands up to kExpandGroup consecutive tiles [t0, t0 + nt) at once.  Every
+// thread loads its bitmap byte of each tile first (all loads in flight
+// together), one workgroup scan ranks the matches of all tiles (the 8
+// per-tile byte popcounts packed 16 bits each into two 64-bit values; a tile
+// has at most 2048 matches, so fields never carry into each other), and
+// emit(tile, key index inside the tile, output slot) runs for every match in
+// key order within its tile.  off_of(r) is the first output slot of tile
+// t0 + r; it is read before the scans, so its latency overlaps theirs.
+template <class OffOf, class Emit>
+__device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
+                                             OffOf&& off_of, uint64_t* sh, Emit&& emit) {
+  ExpandPrep P;
+  expand_prep(bits, t0, nt, off_of, sh, P);
 #pragma unroll
   for (int r = 0; r < kExpandGroup; ++r) {
     if (static_cast<uint32_t>(r) >= nt) break;
-    const uint64_t ex = r < 4 ? elo : ehi;
-    uint32_t o = base[r] + static_cast<uint32_t>((ex >> (16 * (r % 4))) & 0xFFFFu);
-    uint32_t b = b8[r];
+    uint32_t o = P.base[r] + expand_field(P.ex, r);
+    uint32_t b = P.b8[r];
     while (b) {
       const uint32_t j = __ffs(b) - 1u;
-      emit(t0 + r, kb + j, o++);
+      emit(t0 + r, P.kb + j, o++);
       b &= b - 1u;
     }
+  }
+}
+
+// expand_group writing value(tile, key index) to out[slot].  Where the group
+// averages at least kStageMin matches per tile, each tile's values are first
+// placed in LDS (stage, kTileKeys words) and then written out by consecutive
+// lanes to consecutive slots; sparser groups store straight from the bit
+// loop (a lane's few stores are spread over the tile's output, and the
+// barriers of the staged form would cost more than they save).
+constexpr uint32_t kStageMin = 256;
+
+template <class OffOf, class Value>
+__device__ __forceinline__ void expand_group_out(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
+                                                 OffOf&& off_of, uint64_t* sh, uint32_t* stage, Value&& value,
+                                                 uint32_t* __restrict__ out) {
+  ExpandPrep P;
+  expand_prep(bits, t0, nt, off_of, sh, P);
+  uint32_t total = 0;
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) total += static_cast<uint32_t>(r) < nt ? expand_field(P.tot, r) : 0u;
+  if (total < nt * kStageMin) {  // uniform
+#pragma unroll
+    for (int r = 0; r < kExpandGroup; ++r) {
+      if (static_cast<uint32_t>(r) >= nt) break;
+      uint32_t o = P.base[r] + expand_field(P.ex, r);
+      uint32_t b = P.b8[r];
+      while (b) {
+        const uint32_t j = __ffs(b) - 1u;
+        out[o++] = value(t0 + r, P.kb + j);
+        b &= b - 1u;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < kExpandGroup; ++r) {
+    if (static_cast<uint32_t>(r) >= nt) break;  // uniform
+    uint32_t o = expand_field(P.ex, r);
+    uint32_t b = P.b8[r];
+    while (b) {
+      const uint32_t j = __ffs(b) - 1u;
+      stage[o++] = value(t0 + r, P.kb + j);
+      b &= b - 1u;
+    }
+    __syncthreads();
+    const uint32_t cnt = expand_field(P.tot, r);
+    uint32_t* dst = out + P.base[r];
+    for (uint32_t i = threadIdx.x; i < cnt; i += kExpandThreads) dst[i] = stage[i];
+    __syncthreads();
   }
 }
 
@@ -197,11 +272,12 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t
                                                                   uint32_t* __restrict__ out,
                                                                   const uint64_t* __restrict__ d_base) {
   __shared__ uint64_t sh[kExpandThreads / 64];
+  __shared__ uint32_t stage[kTileKeys];
   const uint32_t t0 = blockIdx.x * per_wg;
   uint32_t* dst = out + (d_base ? *d_base : 0ull);
-  expand_group(
-      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh,
-      [&](uint32_t tile, uint32_t i, uint32_t o) { dst[o] = src[static_cast<uint64_t>(tile) * kTileKeys + i]; });
+  expand_group_out(
+      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh, stage,
+      [&](uint32_t tile, uint32_t i) { return src[static_cast<uint64_t>(tile) * kTileKeys + i]; }, dst);
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
@@ -210,10 +286,11 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t
                                                                   uint32_t match_offset,
                                                                   uint32_t* __restrict__ pos) {
   __shared__ uint64_t sh[kExpandThreads / 64];
+  __shared__ uint32_t stage[kTileKeys];
   const uint32_t t0 = blockIdx.x * per_wg;
-  expand_group(
-      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh,
-      [&](uint32_t tile, uint32_t i, uint32_t o) { pos[o] = tile * kTileKeys + i + match_offset; });
+  expand_group_out(
+      bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh, stage,
+      [&](uint32_t tile, uint32_t i) { return tile * kTileKeys + i + match_offset; }, pos);
 }
 
 // Short lists (at most kFusedTiles tiles): the scan folded into the expand.

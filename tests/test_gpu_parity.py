@@ -247,6 +247,29 @@ def test_compaction_at_fused_scan_limit(n):
     assert np.array_equal(D.positions(out, cnt), exp)
 
 
+def test_compaction_dense_and_sparse_groups():
+    """A list of more than 8192 tiles (scan + expand) whose density changes
+    along the batch: runs of 100%, ~25%, ~12% (around the density from which
+    a workgroup stages each tile's output in LDS) and 1% hits, and a last
+    partial tile, so that both store forms of the expand and the switch
+    between neighbouring groups are exercised, against the oracle."""
+    cfg = (32, 1 << 20, 4, 1, 1, 1)
+    desc = D.bbf_desc(*cfg[:1], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5])
+    f = O.Bloom(*cfg)
+    build = np.arange(1, (1 << 16) + 1, dtype=np.uint32) * np.uint32(2654435761 & 0x7FFFFFFF)
+    words = gpu_build(desc, build)
+    parts = []
+    for i, (size, hit) in enumerate([(3 << 20, 1.0), (1 << 21, 0.25), (1 << 21, 0.11), (1 << 21, 0.01),
+                                     (5 << 20, 0.5), ((1 << 22) + 777, 0.12), ((1 << 21) + 5, 0.9)]):
+        parts.append(mixed_keys(build, size, hit, 40 + i))
+    probe = np.concatenate(parts)
+    assert len(probe) > 8192 * 2048
+    out, cnt = D.bbf_probe(desc, words, to_dev(probe), 5)
+    torch.cuda.synchronize()
+    exp = f.batch_contains(oracle_words(f, build), probe, 5)
+    assert np.array_equal(D.positions(out, cnt), exp)
+
+
 def part_params(slice_log2):
     p = D.ProbeParams(0, 0, 0, 0, D.PROBE_FORCE_GLOBAL | D.PROBE_PARTITIONED)
     p.slice_log2 = slice_log2
