@@ -127,16 +127,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tasks(uint32_t* cnt,
 
 constexpr int kExpandThreads = 256;
 static_assert(kExpandThreads * 8 == kTileKeys, "one bitmap byte per thread");
-
-// Expands up to kExpandGroup consecutive tiles [t0, t0 + nt) at once.  Every
-// thread loads its bitmap byte of each tile first (all loads in flight
-// together), one workgroup scan ranks the matches of all tiles (the 8
-// per-tile byte popcounts packed 16 bits each into two 64-bit values; a tile
-// has at most 2048 matches, so fields never carry into each other), and
-// emit(tile, key index inside the tile, output slot) runs for every match in
-// key order within its tile.  off_of(r) is the first output slot of tile
-// t0 + r; it is read before the scans, so its latency overlaps theirs.
-constexpr int kExpandGroup = 8;
+constexpr int kExpandGroup = 8;  // tiles per workgroup of an expand, at most
 
 // The part of an expand that does not depend on where the matches go: each
 // thread's bitmap byte of every tile of the group, and its rank among the
@@ -187,48 +178,34 @@ __device__ __forceinline__ uint32_t expand_field(const uint64_t* v, int r) {
 // together), one workgroup scan ranks the matches of all tiles (the 8
 // per-tile byte popcounts packed 16 bits each into two 64-bit values; a tile
 // has at most 2048 matches, so fields never carry into each other), and
-// emit(tile, key index inside the tile, output slot) runs for every match in
-// key order within its tile.  off_of(r) is the first output slot of tile
-// t0 + r; it is read before the scans, so its latency overlaps theirs.
-template <class OffOf, class Emit>
-__device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
-                                             OffOf&& off_of, uint64_t* sh, Emit&& emit) {
-  ExpandPrep P;
-  expand_prep(bits, t0, nt, off_of, sh, P);
-#pragma unroll
-  for (int r = 0; r < kExpandGroup; ++r) {
-    if (static_cast<uint32_t>(r) >= nt) break;
-    uint32_t o = P.base[r] + expand_field(P.ex, r);
-    uint32_t b = P.b8[r];
-    while (b) {
-      const uint32_t j = __ffs(b) - 1u;
-      emit(t0 + r, P.kb + j, o++);
-      b &= b - 1u;
-    }
-  }
-}
-
-// expand_group writing value(tile, key index) to out[slot].  Where the group
-// averages at least kStageMin matches per tile, each tile's values are first
-// placed in LDS (stage, kTileKeys words) and then written out by consecutive
-// lanes to consecutive slots; sparser groups store straight from the bit
-// loop (a lane's few stores are spread over the tile's output, and the
-// barriers of the staged form would cost more than they save).
+// value(tile, key index inside the tile) is written for every match, in key
+// order within its tile, to out_of(tile)[slot].  off_of(r) is the first
+// output slot of tile t0 + r; it is read before the scans, so its latency
+// overlaps theirs.  STAGE forms: where the group averages at least kStageMin
+// matches per tile, each tile's values are first placed in LDS (stage,
+// kTileKeys words) and then written out by consecutive lanes to consecutive
+// slots; sparser groups store straight from the bit loop (a lane's few
+// stores are spread over the tile's output, and the barriers of the staged
+// form would cost more than they save).  The expands of the short-list and
+// batched compactions do not stage: their 8 KiB of LDS per workgroup made
+// the 4096-filter semi-join plan 3.6% slower (the expands share the CUs with
+// LDS-staged probe launches of other groups) and the plain calls 1.5% slower.
 constexpr uint32_t kStageMin = 256;
 
-template <class OffOf, class Value>
-__device__ __forceinline__ void expand_group_out(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
-                                                 OffOf&& off_of, uint64_t* sh, uint32_t* stage, Value&& value,
-                                                 uint32_t* __restrict__ out) {
+template <bool STAGE, class OffOf, class Value, class OutOf>
+__device__ __forceinline__ void expand_group(const uint64_t* __restrict__ bits, uint32_t t0, uint32_t nt,
+                                             OffOf&& off_of, uint64_t* sh, uint32_t* stage, Value&& value,
+                                             OutOf&& out_of) {
   ExpandPrep P;
   expand_prep(bits, t0, nt, off_of, sh, P);
   uint32_t total = 0;
 #pragma unroll
   for (int r = 0; r < kExpandGroup; ++r) total += static_cast<uint32_t>(r) < nt ? expand_field(P.tot, r) : 0u;
-  if (total < nt * kStageMin) {  // uniform
+  if (!STAGE || total < nt * kStageMin) {  // uniform
 #pragma unroll
     for (int r = 0; r < kExpandGroup; ++r) {
       if (static_cast<uint32_t>(r) >= nt) break;
+      uint32_t* out = out_of(t0 + r);
       uint32_t o = P.base[r] + expand_field(P.ex, r);
       uint32_t b = P.b8[r];
       while (b) {
@@ -239,6 +216,7 @@ __device__ __forceinline__ void expand_group_out(const uint64_t* __restrict__ bi
     }
     return;
   }
+  if constexpr (!STAGE) return;
 #pragma unroll
   for (int r = 0; r < kExpandGroup; ++r) {
     if (static_cast<uint32_t>(r) >= nt) break;  // uniform
@@ -251,7 +229,7 @@ __device__ __forceinline__ void expand_group_out(const uint64_t* __restrict__ bi
     }
     __syncthreads();
     const uint32_t cnt = expand_field(P.tot, r);
-    uint32_t* dst = out + P.base[r];
+    uint32_t* dst = out_of(t0 + r) + P.base[r];
     for (uint32_t i = threadIdx.x; i < cnt; i += kExpandThreads) dst[i] = stage[i];
     __syncthreads();
   }
@@ -275,9 +253,10 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_gather(const uint64_t
   __shared__ uint32_t stage[kTileKeys];
   const uint32_t t0 = blockIdx.x * per_wg;
   uint32_t* dst = out + (d_base ? *d_base : 0ull);
-  expand_group_out(
+  expand_group<true>(
       bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh, stage,
-      [&](uint32_t tile, uint32_t i) { return src[static_cast<uint64_t>(tile) * kTileKeys + i]; }, dst);
+      [&](uint32_t tile, uint32_t i) { return src[static_cast<uint64_t>(tile) * kTileKeys + i]; },
+      [&](uint32_t) { return dst; });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t* __restrict__ bits,
@@ -288,9 +267,9 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_single(const uint64_t
   __shared__ uint64_t sh[kExpandThreads / 64];
   __shared__ uint32_t stage[kTileKeys];
   const uint32_t t0 = blockIdx.x * per_wg;
-  expand_group_out(
+  expand_group<true>(
       bits, t0, min(per_wg, ntiles - t0), [&](uint32_t r) { return offs[t0 + r]; }, sh, stage,
-      [&](uint32_t tile, uint32_t i) { return tile * kTileKeys + i + match_offset; }, pos);
+      [&](uint32_t tile, uint32_t i) { return tile * kTileKeys + i + match_offset; }, [&](uint32_t) { return pos; });
 }
 
 // Short lists (at most kFusedTiles tiles): the scan folded into the expand.
@@ -325,9 +304,9 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_fused(const uint64_t*
     if (t0 + nt == ntiles) *d_count = o;
   }
   __syncthreads();
-  expand_group(
-      bits, t0, nt, [&](uint32_t r) { return s_off[r]; }, sh,
-      [&](uint32_t tile, uint32_t i, uint32_t o) { pos[o] = tile * kTileKeys + i + match_offset; });
+  expand_group<false>(
+      bits, t0, nt, [&](uint32_t r) { return s_off[r]; }, sh, nullptr,
+      [&](uint32_t tile, uint32_t i) { return tile * kTileKeys + i + match_offset; }, [&](uint32_t) { return pos; });
 }
 
 __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t* __restrict__ bits,
@@ -337,12 +316,13 @@ __global__ __launch_bounds__(kExpandThreads) void k_expand_tasks(const uint64_t*
                                                                  uint32_t total_tiles, uint32_t per_wg) {
   __shared__ uint64_t sh[kExpandThreads / 64];
   const uint32_t g0 = blockIdx.x * per_wg;
-  expand_group(
-      bits, g0, min(per_wg, total_tiles - g0), [&](uint32_t r) { return offs[g0 + r]; }, sh,
-      [&](uint32_t g, uint32_t i, uint32_t o) {
+  expand_group<false>(
+      bits, g0, min(per_wg, total_tiles - g0), [&](uint32_t r) { return offs[g0 + r]; }, sh, nullptr,
+      [&](uint32_t g, uint32_t i) {
         const BatchTaskDev& t = tasks[tile_task[g]];
-        t.pos[o] = (g - t.tile_begin) * kTileKeys + i + t.match_offset;
-      });
+        return (g - t.tile_begin) * kTileKeys + i + t.match_offset;
+      },
+      [&](uint32_t g) { return tasks[tile_task[g]].pos; });
 }
 
 __global__ void k_zero_count(uint64_t* d_count) { *d_count = 0; }
