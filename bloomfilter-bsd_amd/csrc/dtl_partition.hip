@@ -618,10 +618,13 @@ __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ s
     const uint64_t v = static_cast<uint64_t>(lbits[2 * w]) | (static_cast<uint64_t>(lbits[2 * w + 1]) << 32);
     bits[static_cast<uint64_t>(tile0) * kTileWords + w] = v;
   }
-  for (uint32_t t = tid; t < tiles; t += 256) {
-    uint32_t c = 0;
-    for (uint32_t j = 0; j < 2 * kTileWords; ++j) c += __popc(lbits[t * 2 * kTileWords + j]);
-    tile_cnt[tile0 + t] = c;
+  // a tile is 2 kTileWords = 64 LDS words: one per lane of a wave (a thread
+  // summing its own tile's words would have every lane of a wave on one bank)
+  static_assert(2 * kTileWords == 64, "one LDS word of a tile per lane");
+  const uint32_t lane = tid & 63u;
+  for (uint32_t t = tid >> 6; t < tiles; t += 256 / 64) {
+    const uint32_t c = wave_sum(__popc(lbits[t * 64u + lane]));
+    if (lane == 0) tile_cnt[tile0 + t] = c;
   }
 }
 
