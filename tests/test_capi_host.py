@@ -29,6 +29,36 @@ def declared_functions():
     return names
 
 
+def header_constants():
+    """name -> value of every `DTL_NAME = value` enumerator in include/*.h."""
+    vals = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", fn)).read(), flags=re.S)
+        for m in re.finditer(r"\b(DTL_[A-Z0-9_]+)\s*=\s*(-?\d+)u?\b", text):
+            vals[m.group(1)] = int(m.group(2))
+    return vals
+
+
+def test_python_constants_match_the_header():
+    """Every status code, addressing mode and probe flag the Python binding
+    names has the header's value, and every probe flag of the header has a
+    Python name."""
+    hv = header_constants()
+    pairs = {"OK": "DTL_OK", "EINVAL": "DTL_EINVAL", "ERUNTIME": "DTL_ERUNTIME", "ELOGIC": "DTL_ELOGIC",
+             "ENOMEM": "DTL_ENOMEM", "EHIP": "DTL_EHIP", "ENODEVICE": "DTL_ENODEVICE",
+             "ADDR_POW2": "DTL_ADDR_POW2", "ADDR_MAGIC": "DTL_ADDR_MAGIC"}
+    flags = [k for k in hv if k.startswith("DTL_PROBE_")]
+    assert len(flags) >= 8
+    for k in flags:
+        pairs[k[len("DTL_"):]] = k
+    for py, h in pairs.items():
+        assert getattr(D, py) == hv[h], (py, h)
+    vals = [hv[k] for k in flags]
+    assert len(set(vals)) == len(vals) and all(v & (v - 1) == 0 for v in vals)
+
+
 def test_library_exports_every_declared_symbol():
     L = D.load()
     names = declared_functions()
