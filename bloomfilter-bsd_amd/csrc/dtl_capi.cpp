@@ -566,6 +566,11 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     if (params && params->ev_start &&
         hipEventRecord(static_cast<hipEvent_t>(params->ev_start), st) != hipSuccess)
       return DTL_EHIP;
+    // a key matches at most once: the hit sort writes the ascending
+    // positions and the count itself (no key-order bitmap, no compaction)
+    P.out_pos = d_pos;
+    P.match_offset = match_offset;
+    P.d_count_out = d_count;
     hipError_t he = partition_keys(P);
     if (he == hipSuccess) he = e->probe_part(P);
     if (he == hipSuccess) he = part_reorder_matches(P);
@@ -573,7 +578,7 @@ int bbf_probe_ex(const dtl_bbf_desc* d, const void* d_words, const uint32_t* d_k
     if (params && params->ev_stop &&
         hipEventRecord(static_cast<hipEvent_t>(params->ev_stop), st) != hipSuccess)
       return DTL_EHIP;
-    return hip_status(compact_single(P.bits, P.tile_cnt, ntiles, P.n, match_offset, d_pos, d_count, st));
+    return DTL_OK;
   }
 
   uint32_t threads = (params && params->block_threads) ? params->block_threads : 0;

@@ -126,6 +126,12 @@ struct BbfPartLaunch {
   const uint64_t* skip_bits;
   const uint64_t* gate_count;
   uint64_t gate_thr;
+  // Blocked Bloom probe (no key can match twice): when set, the hit sort
+  // writes the ascending positions + match_offset here itself and the match
+  // count to *d_count_out, instead of the key-order bitmap and tile counts.
+  uint32_t* out_pos;
+  uint32_t match_offset;
+  uint64_t* d_count_out;
 };
 
 // Stages of the hit sort (part_sort_matches) around a device-side gate
@@ -187,6 +193,9 @@ constexpr uint32_t kPartLanes = 8;    // sub-ranges per partition (one per XCD)
 // (partitioned-order bitmap -> positions -> key-order bitmap + tile counts).
 hipError_t partition_keys(const BbfPartLaunch& L);
 hipError_t part_reorder_matches(const BbfPartLaunch& L);
+// Largest hit-sort bucket whose positions k_hit_bits can write in order (in
+// tiles of kTileKeys): 2^19 positions, the bucket size of 2^32 keys.
+constexpr uint32_t kHitEmitTilesMax = 256;
 // The two halves of part_reorder_matches: positions of the grouped-order
 // matches appended to hp at *d_base (0 if null), their number to *d_count;
 // then the hit sort of hp[0 .. *d_hits) into the key-order bitmap.

@@ -578,12 +578,19 @@ __global__ __launch_bounds__(256) void k_hit_place(const uint32_t* __restrict__ 
 // One workgroup per bucket of 2^shift positions: its matches set bits of an
 // LDS bitmap (dynamic, 2^shift bits), which is written out whole (zeros
 // included) together with the match count of each of its tiles.  The second
-// stage of an open gate starts from the bitmap the first stage wrote.
+// stage of an open gate starts from the bitmap the first stage wrote.  With
+// out_pos (no position can occur twice, so boffsets[b] is the number of
+// matches before the bucket), the bucket's positions are written in
+// ascending order at out_pos + boffsets[b] instead, and workgroup 0 writes
+// the total to *d_count_out.
 __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ sorted,
                                                   const uint32_t* __restrict__ boffsets, uint32_t shift,
                                                   uint32_t ntiles, uint64_t* __restrict__ bits,
-                                                  uint32_t* __restrict__ tile_cnt, HitSel sel) {
+                                                  uint32_t* __restrict__ tile_cnt, HitSel sel,
+                                                  uint32_t* __restrict__ out_pos, uint32_t match_offset,
+                                                  uint64_t* __restrict__ d_count_out) {
   extern __shared__ uint32_t lbits[];
+  __shared__ uint32_t ctot[kHitEmitTilesMax];
   if (!hit_stage_runs(sel)) return;
   const bool acc = sel.stage == kHitSortRest && hit_gate_open(sel);
   const uint32_t words32 = (1u << shift) / 32u;
@@ -616,15 +623,57 @@ __global__ __launch_bounds__(256) void k_hit_bits(const uint32_t* __restrict__ s
     atomicOr(&lbits[p >> 5], 1u << (p & 31u));
   }
   __syncthreads();
+  static_assert(2 * kTileWords == 64, "one LDS word of a tile per lane");
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  if (out_pos) {  // uniform
+    // tile c of the bucket: lane l holds its LDS word l; tile totals, one
+    // exclusive scan of them (wave 0, 4 tiles per lane), then every wave
+    // writes its tiles' positions at the tile's base + the lane's rank
+    for (uint32_t c = wave; c < tiles; c += 256 / 64) {
+      const uint32_t v = wave_sum(__popc(lbits[c * 64u + lane]));
+      if (lane == 0) ctot[c] = v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      static_assert(kHitEmitTilesMax == 4 * 64, "4 tile totals per lane");
+      uint32_t a[4], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t c = lane * 4u + k;
+        a[k] = c < tiles ? ctot[c] : 0u;
+        sum += a[k];
+      }
+      uint32_t run = wave_incl_scan(sum) - sum;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t c = lane * 4u + k;
+        if (c < tiles) ctot[c] = run;
+        run += a[k];
+      }
+    }
+    __syncthreads();
+    const uint32_t base = boffsets[b];
+    for (uint32_t c = wave; c < tiles; c += 256 / 64) {
+      uint32_t w = lbits[c * 64u + lane];
+      const uint32_t pc = __popc(w);
+      uint32_t o = base + ctot[c] + wave_incl_scan(pc) - pc;
+      const uint32_t p0 = (tile0 + c) * kTileKeys + lane * 32u + match_offset;
+      while (w) {
+        const uint32_t j = __ffs(w) - 1u;
+        out_pos[o++] = p0 + j;
+        w &= w - 1u;
+      }
+    }
+    if (b == 0 && tid == 0) *d_count_out = *sel.d_hits;
+    return;
+  }
   for (uint32_t w = tid; w < tiles * kTileWords; w += 256) {
     const uint64_t v = static_cast<uint64_t>(lbits[2 * w]) | (static_cast<uint64_t>(lbits[2 * w + 1]) << 32);
     bits[static_cast<uint64_t>(tile0) * kTileWords + w] = v;
   }
   // a tile is 2 kTileWords = 64 LDS words: one per lane of a wave (a thread
   // summing its own tile's words would have every lane of a wave on one bank)
-  static_assert(2 * kTileWords == 64, "one LDS word of a tile per lane");
-  const uint32_t lane = tid & 63u;
-  for (uint32_t t = tid >> 6; t < tiles; t += 256 / 64) {
+  for (uint32_t t = wave; t < tiles; t += 256 / 64) {
     const uint32_t c = wave_sum(__popc(lbits[t * 64u + lane]));
     if (lane == 0) tile_cnt[tile0 + t] = c;
   }
@@ -770,8 +819,11 @@ hipError_t part_sort_matches(const BbfPartLaunch& L, const uint32_t* hp, const u
   // bucketed positions overwrite the grouped positions
   hipLaunchKernelGGL(k_hit_place, dim3(G), dim3(256), nb * 8u, st, hp, sel, L.bucket_shift, nb,
                      L.hcounts, L.boffsets, L.pos_p);
+  uint32_t* out_pos = stage == kHitSortAll ? L.out_pos : nullptr;
+  if (out_pos && ((1u << L.bucket_shift) / kTileKeys > kHitEmitTilesMax || !L.d_count_out))
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_hit_bits, dim3(nb), dim3(256), (1u << L.bucket_shift) / 8u, st, L.pos_p, L.boffsets,
-                     L.bucket_shift, L.ntiles, L.bits, L.tile_cnt, sel);
+                     L.bucket_shift, L.ntiles, L.bits, L.tile_cnt, sel, out_pos, L.match_offset, L.d_count_out);
   return hipGetLastError();
 }
 
