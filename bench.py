@@ -968,7 +968,8 @@ def main():
             "traffic_measured_at": traffic.get("commit") if traffic else None,
             "kernel": ("partitioned probe phase (k_part_init_bins, k_part_scatter into fixed-capacity "
                        "bins, k_bbf_probe_split over the grouped keys: 4 lanes per key, one 16 B piece "
-                       "of the 64 B block each; match reorder), HIP events around the phase" if partitioned
+                       "of the 64 B block each; match reorder, whose hit sort writes the ascending positions), HIP "
+                       "events around the phase" if partitioned
                        else "k_bbf_probe_global<u64,w8,s8,k8>"),
             "kernel_ms": round(kern_ms, 4),
             "kernel_ms_per_call": per_call,
