@@ -24,8 +24,9 @@
 //      scan + k_expand_gather list the original positions of the matches,
 //      k_hit_count / k_hit_colscan / k_part_scan / k_hit_place bucket them by
 //      position (2^18 positions per bucket or more), and k_hit_bits builds
-//      each bucket's piece of the key-order bitmap in LDS and writes it with
-//      its tile counts.
+//      each bucket's piece of the key-order bitmap in LDS and writes either
+//      the bucket's positions in ascending order (blocked Bloom filters: a key
+//      matches once) or the bitmap piece with its tile counts (cuckoo).
 // The order of keys inside a partition depends on atomic timing, but the
 // key-order bitmap does not, so the match list is the same ascending list as
 // the direct probe's.
