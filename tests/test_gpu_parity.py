@@ -976,3 +976,64 @@ def test_scratch_growth_after_capture_is_refused():
     torch.cuda.synchronize()
     assert np.array_equal(D.positions(out, cnt), f.batch_contains(ow, large, 0))
     D.scratch_release(s)
+
+
+def test_partitioned_probes_replay_from_a_graph():
+    """The partitioned probes decide on the device alone (the cuckoo probe's
+    gate on the first pass's match count, the Bloom probe's hit sort writing
+    the positions and the count): captured into one hipGraph -- the cuckoo
+    table at 1% and 50% hits (gate closed and open) and a Bloom filter --
+    every replay, with eager work between replays, gives the oracle's lists."""
+    m = (1 << 20) * 64
+    cd = D.cf_desc(m, 16, 4)
+    co = O.Cuckoo(m, 16, 4)
+    ckeys = distinct_keys(int(0.95 * cd.bucket_cnt * 4), 51)
+    cbuf, ins = D.cf_build_host(cd, ckeys, seed=51)
+    assert ins == len(ckeys)
+    n = (1 << 21) + 5
+    cprobes = [mixed_keys(ckeys, n, hit, 52 + i) for i, hit in enumerate((0.01, 0.5))]
+    cexp = [co.batch_contains(cbuf, p, 9) for p in cprobes]
+    cfg = (64, 1 << 25, 8, 8, 8, 1)
+    bd = D.bbf_desc(*cfg)
+    bo = O.Bloom(*cfg)
+    bkeys = distinct_keys(1 << 18, 53)
+    words = gpu_build(bd, bkeys)
+    bprobe = mixed_keys(bkeys, n, 0.05, 54)
+    bexp = bo.batch_contains(oracle_words(bo, bkeys), bprobe, 11)
+    dbuf = torch.from_numpy(cbuf).to(dev())
+    ckd = [to_dev(p) for p in cprobes]
+    bkd = to_dev(bprobe)
+    cp = D.ProbeParams(0, 0, 0, 0, D.PROBE_PARTITIONED)
+    cp.slice_log2 = 19      # 16 partitions with fixed-capacity bins
+    bp = D.ProbeParams(0, 0, 0, 0, D.PROBE_PARTITIONED)
+    bp.slice_log2 = 17      # 32 partitions with fixed-capacity bins
+    outs = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+    cnts = [torch.zeros(1, dtype=torch.int64, device="cuda") for _ in range(3)]
+    s = torch.cuda.Stream()
+
+    def calls():
+        for i in range(2):
+            D.cf_probe(cd, dbuf, ckd[i], 9, stream=s, out=outs[i], count=cnts[i], params=cp)
+        D.bbf_probe(bd, words, bkd, 11, stream=s, out=outs[2], count=cnts[2], params=bp)
+
+    with torch.cuda.stream(s):
+        calls()       # sizes the stream's scratch before the capture
+    torch.cuda.synchronize()
+    exp = cexp + [bexp]
+    for i in range(3):
+        assert np.array_equal(D.positions(outs[i], cnts[i]), exp[i]), ("eager", i)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        calls()
+    for rep in range(3):
+        for o, c in zip(outs, cnts):
+            o.fill_(-1)
+            c.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i in range(3):
+            assert np.array_equal(D.positions(outs[i], cnts[i]), exp[i]), ("replay", rep, i)
+    del g
+    torch.cuda.synchronize()
+    D.scratch_release(s)
