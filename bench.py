@@ -804,15 +804,17 @@ def summarize_traffic(fetch_csv, write_csv, commit):
             for i in range(first, last + 1)]
     fb, wb = sum(k["fetch_bytes"] for k in kern), sum(k["write_bytes"] for k in kern)
     d = [i for i, x in enumerate(f) if "k_bbf_probe_global<" in x[1]][-1]
+    fdir, wdir = os.path.dirname(fetch_csv) or ".", os.path.dirname(write_csv) or "."
     return {
         "workload": "cache-sectorized BBF u64 w8 s8 k8, 1 GiB filter, 2^28 probe keys at 1% hits (bench.py defaults)",
         "commit": commit,
-        "commands": ["rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pf -o pf --output-format csv -- "
-                     "python3 bench.py --steps 2 --warmup 1 --no-cpu --headline-only --calib-copy",
-                     "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pw -o pw --output-format csv -- "
-                     "python3 bench.py --steps 2 --warmup 1 --no-cpu --headline-only --calib-copy",
-                     "python3 bench.py --summarize-traffic gpurun_out/pf/pf_counter_collection.csv "
-                     "gpurun_out/pw/pw_counter_collection.csv --commit <measured commit>"],
+        "commands": ["rocprofv3 --pmc FETCH_SIZE -d %s -o %s --output-format csv -- python3 bench.py --steps 2 "
+                     "--warmup 1 --no-cpu --headline-only --calib-copy"
+                     % (fdir, os.path.basename(fetch_csv).split("_counter_collection")[0]),
+                     "rocprofv3 --pmc WRITE_SIZE -d %s -o %s --output-format csv -- python3 bench.py --steps 2 "
+                     "--warmup 1 --no-cpu --headline-only --calib-copy"
+                     % (wdir, os.path.basename(write_csv).split("_counter_collection")[0]),
+                     "python3 bench.py --summarize-traffic %s %s --commit %s" % (fetch_csv, write_csv, commit)],
         "copy_calibration": {"copy_bytes": copy_bytes, "fetch_kib": f[ci][2] / 1024, "write_kib": w[ci][2] / 1024,
                              "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)},
         "partitioned_phase": {"kernels": kern, "fetch_bytes": fb, "write_bytes": wb},

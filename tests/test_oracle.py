@@ -259,7 +259,7 @@ def test_k8_specialization_quirk():
     for cfg in [(32, 1 << 20, 4, 1, 1, 1), (64, 1 << 20, 8, 8, 8, 1), (32, 10240, 6, 16, 16, 2),
                 (64, 1 << 20, 8, 1, 1, 1)]:
         assert not O.Bloom(*cfg).has_k8_quirk(), cfg
-    # affected configurations named in the survey of the reference
+    # configurations of the reference whose k = 8 path keeps the stale hash state
     for cfg in [(32, 1 << 16, 16, 2, 8, 1), (64, 1 << 16, 16, 2, 2, 1), (64, 1 << 16, 16, 2, 4, 1),
                 (64, 1 << 16, 16, 2, 16, 1), (32, 1 << 16, 16, 16, 16, 2)]:
         assert O.Bloom(*cfg).has_k8_quirk(), cfg
